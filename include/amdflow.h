@@ -1,0 +1,181 @@
+/* amdflow.h -- C ABI of the MI355X optical-flow library (libamdflow.so).
+ *
+ * Drop-in boundary for the reference's three per-algorithm entry points
+ * (artecs-group/sycl-optic-flow).  Every function returns an of_status
+ * (0 = OK) and never throws or exits.  Handles own all device memory; the
+ * caller owns the host arrays it passes.  Calls on one handle must come from
+ * one host thread at a time.  All *_run/compute calls taking host pointers
+ * are synchronous: results are in host memory when they return, as in the
+ * reference (src/tvl1/cuda/tvl1.cu:181-182, src/hs/cuda/flow.cu:206-207).
+ * The *_device variants take device pointers and a HIP stream (passed as
+ * void*) and return once the work is complete on that stream.
+ */
+#ifndef AMDFLOW_H
+#define AMDFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    OF_OK = 0,
+    OF_ERR_ARG = 1,          /* invalid size or parameter */
+    OF_ERR_SIGMA = 2,        /* Gaussian window larger than an image level
+                                (the reference's "GaussianSmooth: sigma too
+                                large", mask.cpp:231-234) */
+    OF_ERR_ALLOC = 3,        /* device or host allocation failed */
+    OF_ERR_HIP = 4,          /* a HIP runtime call failed */
+    OF_ERR_STATE = 5,        /* call not valid in the handle's state */
+    OF_ERR_UNSUPPORTED = 6   /* filter size or option not provided */
+} of_status;
+
+const char *of_status_string(int status);
+/* Library build identification (architecture, flags). */
+const char *of_build_info(void);
+
+/* ------------------------------------------------------------------------
+ * TV-L1 (Zach-Pock-Bischof dual TV-L1, coarse to fine).
+ * Replaces class TV_L1, src/tvl1/sequential/tvl1.hpp:10-41.
+ * ---------------------------------------------------------------------- */
+typedef struct of_tvl1 of_tvl1;
+
+typedef struct {
+    float tau;          /* time step                       (tvl1.hpp:12) */
+    float lambda;       /* data-term weight                (tvl1.hpp:12) */
+    float theta;        /* coupling weight                 (tvl1.hpp:13) */
+    int nscales;        /* requested scales (upper bound)  (tvl1.hpp:13) */
+    float zfactor;      /* pyramid zoom factor in (0,1)    (tvl1.hpp:13) */
+    int warps;          /* warpings per scale              (tvl1.hpp:13) */
+    float epsilon;      /* stopping threshold              (tvl1.hpp:14) */
+    int max_iterations; /* per-warping cap, MAX_ITERATIONS (tvl1.hpp:6)  */
+} of_tvl1_params;
+
+/* Fills the reference defaults: tau 0.25, lambda 0.15, theta 0.3,
+ * nscales 100, zfactor 0.5, warps 5, epsilon 0.01, max_iterations 300
+ * (src/tvl1/sequential/tvl1.hpp:6,12-14). */
+void of_tvl1_default_params(of_tvl1_params *p);
+
+/* Constructor: TV_L1::TV_L1(int width, int height, float tau, ...)
+ * (src/tvl1/sequential/tvl1.cpp:69-121).  The number of scales follows the
+ * reference rule min(nscales, 1 + ln(hypot(W,H)/16)/ln(1/zfactor))
+ * (tvl1.cpp:84-85), clamped to >= 1.  `params` may be NULL for defaults. */
+int of_tvl1_create(const of_tvl1_params *params, int width, int height, of_tvl1 **out);
+void of_tvl1_destroy(of_tvl1 *h);
+
+/* Number of pyramid scales in use and the size of scale `s` (0 = finest). */
+int of_tvl1_nscales(const of_tvl1 *h);
+int of_tvl1_scale_size(const of_tvl1 *h, int s, int *nx, int *ny);
+
+/* Pair solve on 8-bit frames: TV_L1::runDualTVL1Multiscale(uint8_t* I0,
+ * uint8_t* I1) (src/tvl1/sequential/tvl1.hpp:17, tvl1.cpp:171-217).
+ * Host pointers of width*height bytes each.  Flow available through
+ * of_tvl1_flow. */
+int of_tvl1_run_u8(of_tvl1 *h, const uint8_t *I0, const uint8_t *I1);
+
+/* Same solve on float frames (host pointers, width*height floats each);
+ * the u8->float cast of tvl1.cpp:160-165 is skipped. */
+int of_tvl1_run_f32(of_tvl1 *h, const float *I0, const float *I1);
+
+/* Streaming solve: TV_L1::runDualTVL1Multiscale(const float* I) of the GPU
+ * variants (src/tvl1/cuda/tvl1.cuh:19, tvl1.cu:122-183): I0 is the frame
+ * passed on the previous call, I1 the frame passed now.  The first call has
+ * no previous frame: it stores I and leaves the flow at zero. */
+int of_tvl1_push_f32(of_tvl1 *h, const float *I);
+
+/* getU(): planar [u1 | u2], 2*width*height floats, owned by the handle and
+ * valid until the next run or destroy (src/tvl1/sequential/tvl1.hpp:16). */
+const float *of_tvl1_flow(of_tvl1 *h);
+
+/* Device-resident variant used by throughput measurements and by callers
+ * that keep frames in HBM: dI0/dI1 are device pointers to width*height
+ * bytes, dU a device pointer to 2*width*height floats (planar [u1|u2]).
+ * `stream` is a hipStream_t (NULL = the handle's own stream).  Returns after
+ * the solve has completed on that stream. */
+int of_tvl1_run_device_u8(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, float *dU,
+                          void *stream);
+int of_tvl1_run_device_f32(of_tvl1 *h, const float *dI0, const float *dI1, float *dU,
+                           void *stream);
+
+/* Per-(scale, warping) iteration counts of the last solve, finest scale
+ * first: counts[s*warps + w].  `cap` is the array length. */
+int of_tvl1_iterations(const of_tvl1 *h, int *counts, int cap);
+
+typedef struct {
+    int nscales;
+    int warps;
+    long long pixel_iterations;    /* sum over (scale,warp) of iters*N_s    */
+    long long pixel_warps;         /* warps * sum of N_s                     */
+    long long pixels_all_scales;   /* sum of N_s                             */
+    long long launches;            /* iteration-kernel launches issued       */
+    double algorithmic_bytes;      /* 60*PI + 36*PW + 64*sum N_s + 48*N_0    */
+} of_tvl1_stats;
+int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
+
+/* Iteration-kernel launch shape for pyramid scales whose pixel count is at
+ * least `min_pixels`: `iters_per_launch` primal-dual iterations run in one
+ * launch on tiles of 64 x (4*rows_per_wave) pixels with a halo of
+ * iters_per_launch pixels.  Supported: iters_per_launch in {1,2,4,8},
+ * rows_per_wave in {2,4,8}.  Rules are matched largest min_pixels first;
+ * calling with min_pixels < 0 clears all rules (back to the built-in table). */
+int of_tvl1_set_launch_rule(of_tvl1 *h, long long min_pixels, int iters_per_launch,
+                            int rows_per_wave);
+
+/* ------------------------------------------------------------------------
+ * Horn-Schunck.  Replaces initFlow / ComputeFlow / deleteFlow_mem,
+ * src/hs/cuda/flow.cuh:19-33 (global state there; a handle here).
+ * ---------------------------------------------------------------------- */
+typedef struct of_hs of_hs;
+
+/* initFlow(int nLevels, int stride, int width, int height)
+ * (src/hs/cuda/flow.cu:46-72). */
+int of_hs_create(int nLevels, int stride, int width, int height, of_hs **out);
+/* deleteFlow_mem(int nLevels) (src/hs/cuda/flow.cu:74-98). */
+void of_hs_destroy(of_hs *h);
+/* ComputeFlow(I0, I1, width, height, stride, alpha, nLevels, nWarpIters,
+ * nSolverIters, u, v) (src/hs/cuda/flow.cu:112-208).  Host pointers with
+ * row stride `stride` (floats); u, v receive stride*height floats. */
+int of_hs_compute(of_hs *h, const float *I0, const float *I1, int width, int height, int stride,
+                  float alpha, int nLevels, int nWarpIters, int nSolverIters, float *u, float *v);
+/* Device-pointer variant (same layout), synchronous on `stream`. */
+int of_hs_compute_device(of_hs *h, const float *dI0, const float *dI1, int width, int height,
+                         int stride, float alpha, int nLevels, int nWarpIters, int nSolverIters,
+                         float *du, float *dv, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Lucas-Kanade.  Replaces class LucasKanade, src/lk/cuda/lucaskanade.cuh:6-35.
+ * ---------------------------------------------------------------------- */
+typedef struct of_lk of_lk;
+
+/* LucasKanade(int spac_filt_size, int temp_filt_size, int window_size,
+ * int nx, int ny) (src/lk/cuda/lucaskanade.cu:5-32).  Spatial filter sizes
+ * 1/3/5/7/9, temporal 1/2/3/5/7/9 (lucaskanade.cu:79-161). */
+int of_lk_create(int spac_filt_size, int temp_filt_size, int window_size, int nx, int ny,
+                 of_lk **out);
+void of_lk_destroy(of_lk *h);
+/* copy_frames_circularbuffer_GPU_wrapper(frame, temp_size, fr, frame_size)
+ * (src/lk/cuda/lucaskanade.cu:74-76, kernels.cu:6-10): frame `fr` goes into
+ * slot fr % temp_size of the device ring. */
+int of_lk_push_u8(of_lk *h, const uint8_t *frame, int temp_size, int fr, int frame_size);
+/* lucas_kanade(float* Vx, float* Vy, int iframe) (lucaskanade.cu:54-71):
+ * host outputs of nx*ny floats. */
+int of_lk_compute(of_lk *h, float *Vx, float *Vy, int iframe);
+
+/* Pyramidal iterative LK on a float pair (no reference counterpart: the
+ * reference's LK is single-scale).  `levels` Gaussian-pyramid levels
+ * (blur `sigma`, decimation by `zf`), `window` x `window` sums, `iters`
+ * iterations per level, update only where det(G) > det_eps.  Host pointers;
+ * u, v receive width*height floats. */
+int of_lk_pyr_compute(const float *I0, const float *I1, int width, int height, int levels,
+                      float zf, float sigma, int window, int iters, float det_eps, float *u,
+                      float *v);
+int of_lk_pyr_compute_device(const float *dI0, const float *dI1, int width, int height,
+                             int levels, float zf, float sigma, int window, int iters,
+                             float det_eps, float *du, float *dv, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,46 @@
+/* amdflow_debug.h -- per-kernel entry points of libamdflow.so for parity
+ * tests.  Each takes host arrays, runs one GPU kernel (or one fixed kernel
+ * sequence) on them and copies the result back; they exist so that every
+ * stage of the solver path can be compared with its CPU counterpart.  They
+ * allocate device memory per call and are not meant for production use.
+ * Sizes are width x height row-major float planes unless noted. */
+#ifndef AMDFLOW_DEBUG_H
+#define AMDFLOW_DEBUG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* gaussian(I, xdim, ydim, sigma, buffer), mask.cpp:217-287 (out of place). */
+int of_dbg_gaussian(const float *in, float *out, int width, int height, double sigma);
+/* zoom_out(I, Iout, nx, ny, factor, ...), zoom.cpp:20-53. */
+int of_dbg_zoom_out(const float *in, float *out, int width, int height, float factor);
+/* zoom_in(I, Iout, nx, ny, nxx, nyy), zoom.cpp:61-85, times `scale`. */
+int of_dbg_zoom_in(const float *in, float *out, int width, int height, int nxx, int nyy,
+                   float scale);
+/* centered_gradient, mask.cpp:152-209. */
+int of_dbg_centered_gradient(const float *in, float *dx, float *dy, int width, int height);
+/* image_normalization of two frames, tvl1.cpp:344-371 (in place). */
+int of_dbg_normalize(float *I0, float *I1, int width, int height);
+/* warp set-up of one warping, tvl1.cpp:237-254: I1wx, I1wy, rho_c. */
+int of_dbg_warp_setup(const float *I0, const float *I1, const float *I1x, const float *I1y,
+                      const float *u1, const float *u2, float *I1wx, float *I1wy, float *rho_c,
+                      int width, int height);
+/* `iters` primal-dual iterations (tvl1.cpp:260-333) from the given state,
+ * with launch shape (iters_per_launch, rows_per_wave) and no stopping
+ * test; state arrays are updated in place; err_sums receives the per
+ * iteration sum of (du1^2+du2^2) (double). */
+int of_dbg_tvl1_iterations(const float *I1wx, const float *I1wy, const float *rho_c, float *u1,
+                           float *u2, float *p11, float *p12, float *p21, float *p22, int width,
+                           int height, float tau, float lambda, float theta, int iters,
+                           int iters_per_launch, int rows_per_wave, double *err_sums);
+/* hypot as the solver evaluates it on the GPU, element-wise. */
+int of_dbg_hypot(const float *a, const float *b, float *out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,231 @@
+"""Python binding of libamdflow.so (the MI355X optical-flow library).
+
+The classes mirror the reference's per-algorithm entry points so that code
+written against the reference reads the same:
+
+  TV_L1(width, height, tau=0.25, lambda_=0.15, theta=0.3, nscales=100,
+        zfactor=0.5, warps=5, epsilon=0.01)       src/tvl1/sequential/tvl1.hpp:12-17
+      .runDualTVL1Multiscale(I0, I1)  (uint8 pair) / (float frame, streaming)
+      .getU()                         planar [u1 | u2]
+  HornSchunck: initFlow / ComputeFlow / deleteFlow_mem   src/hs/cuda/flow.cuh:19-33
+  LucasKanade(spac, temp, window, nx, ny)                src/lk/cuda/lucaskanade.cuh:6-35
+      .copy_frames_circularbuffer_GPU_wrapper(frame, temp_size, fr, frame_size)
+      .lucas_kanade(Vx, Vy, iframe)
+
+Errors raise OpticalFlowError with the library's status code (the reference
+throws or terminates).  The library has no CPU fallback: if libamdflow.so is
+missing or cannot be loaded, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libamdflow.so")
+
+
+class OpticalFlowError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {status_string(status)} (status {status})")
+        self.status = status
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() "
+            "(the HIP library is required, there is no CPU fallback)")
+    return C.CDLL(LIB_PATH)
+
+
+_lib = _load()
+
+
+class TVL1Params(C.Structure):
+    _fields_ = [
+        ("tau", C.c_float), ("lambda_", C.c_float), ("theta", C.c_float),
+        ("nscales", C.c_int), ("zfactor", C.c_float), ("warps", C.c_int),
+        ("epsilon", C.c_float), ("max_iterations", C.c_int),
+    ]
+
+
+class TVL1Stats(C.Structure):
+    _fields_ = [
+        ("nscales", C.c_int), ("warps", C.c_int),
+        ("pixel_iterations", C.c_longlong), ("pixel_warps", C.c_longlong),
+        ("pixels_all_scales", C.c_longlong), ("launches", C.c_longlong),
+        ("algorithmic_bytes", C.c_double),
+    ]
+
+
+_vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+# exported symbols and their ctypes signatures (restype, argtypes)
+SIGNATURES = {
+    "of_status_string": (C.c_char_p, [C.c_int]),
+    "of_build_info": (C.c_char_p, []),
+    "of_tvl1_default_params": (None, [C.POINTER(TVL1Params)]),
+    "of_tvl1_create": (C.c_int, [C.POINTER(TVL1Params), C.c_int, C.c_int, C.POINTER(_vp)]),
+    "of_tvl1_destroy": (None, [_vp]),
+    "of_tvl1_nscales": (C.c_int, [_vp]),
+    "of_tvl1_scale_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "of_tvl1_run_u8": (C.c_int, [_vp, _vp, _vp]),
+    "of_tvl1_run_f32": (C.c_int, [_vp, _vp, _vp]),
+    "of_tvl1_push_f32": (C.c_int, [_vp, _vp]),
+    "of_tvl1_flow": (_fp, [_vp]),
+    "of_tvl1_run_device_u8": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "of_tvl1_run_device_f32": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "of_tvl1_iterations": (C.c_int, [_vp, _ip, C.c_int]),
+    "of_tvl1_get_stats": (C.c_int, [_vp, C.POINTER(TVL1Stats)]),
+    "of_tvl1_set_launch_rule": (C.c_int, [_vp, C.c_longlong, C.c_int, C.c_int]),
+    "of_hs_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "of_hs_destroy": (None, [_vp]),
+    "of_hs_compute": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                C.c_int, C.c_int, _vp, _vp]),
+    "of_hs_compute_device": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                       C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "of_lk_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "of_lk_destroy": (None, [_vp]),
+    "of_lk_push_u8": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "of_lk_compute": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "of_lk_pyr_compute": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                    C.c_int, C.c_int, C.c_float, _vp, _vp]),
+    "of_lk_pyr_compute_device": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                           C.c_float, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    if hasattr(_lib, _name):
+        _f = getattr(_lib, _name)
+        _f.restype = _res
+        _f.argtypes = _args
+
+
+def status_string(status: int) -> str:
+    return _lib.of_status_string(int(status)).decode()
+
+
+def build_info() -> str:
+    return _lib.of_build_info().decode()
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise OpticalFlowError(status, what)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _as(a, dtype, shape=None) -> np.ndarray:
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    if shape is not None and arr.size != int(np.prod(shape)):
+        raise ValueError(f"expected {int(np.prod(shape))} elements, got {arr.size}")
+    return arr
+
+
+def default_tvl1_params() -> TVL1Params:
+    p = TVL1Params()
+    _lib.of_tvl1_default_params(C.byref(p))
+    return p
+
+
+class TV_L1:
+    """TV-L1 solver handle; mirrors class TV_L1 of src/tvl1/sequential/tvl1.hpp:10-41.
+
+    runDualTVL1Multiscale(I0, I1) with two uint8 frames solves the pair
+    (tvl1.hpp:17); with one float frame it is the streaming call of the GPU
+    variants, I0 being the previous call's frame (src/tvl1/cuda/tvl1.cuh:19).
+    """
+
+    def __init__(self, width: int, height: int, tau: float = 0.25, lambda_: float = 0.15,
+                 theta: float = 0.3, nscales: int = 100, zfactor: float = 0.5, warps: int = 5,
+                 epsilon: float = 0.01, max_iterations: int = 300):
+        self.width, self.height = int(width), int(height)
+        p = TVL1Params(tau, lambda_, theta, nscales, zfactor, warps, epsilon, max_iterations)
+        self.params = p
+        h = _vp()
+        _check(_lib.of_tvl1_create(C.byref(p), self.width, self.height, C.byref(h)), "TV_L1")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.of_tvl1_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    @property
+    def nscales(self) -> int:
+        return int(_lib.of_tvl1_nscales(self._h))
+
+    def scale_size(self, s: int) -> tuple[int, int]:
+        nx, ny = C.c_int(), C.c_int()
+        _check(_lib.of_tvl1_scale_size(self._h, s, C.byref(nx), C.byref(ny)), "scale_size")
+        return nx.value, ny.value
+
+    def runDualTVL1Multiscale(self, I0, I1=None) -> None:
+        n = self.width * self.height
+        if I1 is None:
+            f = _as(I0, np.float32, (n,))
+            _check(_lib.of_tvl1_push_f32(self._h, _ptr(f)), "runDualTVL1Multiscale")
+            return
+        if np.asarray(I0).dtype == np.uint8 and np.asarray(I1).dtype == np.uint8:
+            a, b = _as(I0, np.uint8, (n,)), _as(I1, np.uint8, (n,))
+            _check(_lib.of_tvl1_run_u8(self._h, _ptr(a), _ptr(b)), "runDualTVL1Multiscale")
+        else:
+            a, b = _as(I0, np.float32, (n,)), _as(I1, np.float32, (n,))
+            _check(_lib.of_tvl1_run_f32(self._h, _ptr(a), _ptr(b)), "runDualTVL1Multiscale")
+
+    def getU(self) -> np.ndarray:
+        n = self.width * self.height
+        p = _lib.of_tvl1_flow(self._h)
+        return np.ctypeslib.as_array(p, shape=(2 * n,)).copy()
+
+    def flow(self) -> tuple[np.ndarray, np.ndarray]:
+        u = self.getU()
+        n = self.width * self.height
+        return (u[:n].reshape(self.height, self.width), u[n:].reshape(self.height, self.width))
+
+    def run_device_u8(self, d_I0: int, d_I1: int, d_U: int, stream: int = 0) -> None:
+        _check(_lib.of_tvl1_run_device_u8(self._h, d_I0, d_I1, d_U, stream or None),
+               "run_device_u8")
+
+    def run_device_f32(self, d_I0: int, d_I1: int, d_U: int, stream: int = 0) -> None:
+        _check(_lib.of_tvl1_run_device_f32(self._h, d_I0, d_I1, d_U, stream or None),
+               "run_device_f32")
+
+    def iterations(self) -> np.ndarray:
+        ns, w = self.nscales, int(self.params.warps)
+        out = np.zeros(ns * w, dtype=np.int32)
+        _check(_lib.of_tvl1_iterations(self._h, out.ctypes.data_as(_ip), out.size), "iterations")
+        return out.reshape(ns, w)
+
+    def stats(self) -> dict:
+        st = TVL1Stats()
+        _check(_lib.of_tvl1_get_stats(self._h, C.byref(st)), "stats")
+        return {f: getattr(st, f) for f, _ in TVL1Stats._fields_}
+
+    def set_launch_rule(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int) -> None:
+        _check(_lib.of_tvl1_set_launch_rule(self._h, int(min_pixels), int(iters_per_launch),
+                                            int(rows_per_wave)), "set_launch_rule")
+
+    def clear_launch_rules(self) -> None:
+        _check(_lib.of_tvl1_set_launch_rule(self._h, -1, 1, 2), "set_launch_rule")

@@ -1,0 +1,69 @@
+// common.hpp -- host-side helpers shared by the TV-L1, HS and LK modules.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "amdflow.h"
+
+namespace amdflow {
+
+// Evaluates a HIP call; on failure records it and returns OF_ERR_HIP from
+// the enclosing function.
+#define OF_HIP(call)                                                     \
+    do {                                                                 \
+        hipError_t of_e_ = (call);                                       \
+        if (of_e_ != hipSuccess) {                                       \
+            ::amdflow::record_hip_error(of_e_, #call, __FILE__, __LINE__); \
+            return OF_ERR_HIP;                                           \
+        }                                                                \
+    } while (0)
+
+#define OF_TRY(expr)                     \
+    do {                                 \
+        int of_s_ = (expr);              \
+        if (of_s_ != OF_OK) return of_s_; \
+    } while (0)
+
+void record_hip_error(hipError_t e, const char* what, const char* file, int line);
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Device allocation that reports failure as a status code.
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return OF_ERR_ALLOC;
+    }
+    return OF_OK;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Gaussian taps of mask.cpp:225-244 computed on the host in double with the
+// C library's exp, so CPU and GPU blur with identical coefficients.
+struct GaussTaps {
+    double B[64];
+    int size;
+};
+int gaussian_taps(double sigma, GaussTaps* g);
+
+// Pyramid size rule of zoom.hpp:14-26.
+inline void zoom_size(int nx, int ny, float factor, int* nxx, int* nyy)
+{
+    *nxx = (int)((double)((float)nx * factor) + 0.5);
+    *nyy = (int)((double)((float)ny * factor) + 0.5);
+}
+
+}  // namespace amdflow

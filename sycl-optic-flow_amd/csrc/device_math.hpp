@@ -1,0 +1,97 @@
+// device_math.hpp -- per-pixel arithmetic shared by the TV-L1, HS and LK
+// kernels.  Every expression is written with the operand types and the
+// evaluation order of the reference's sequential path so that, compiled with
+// -ffp-contract=off (no fused multiply-add) and IEEE fp32 division/sqrt, the
+// GPU produces the same bits as the CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace amdflow {
+
+// Clamp used by the bicubic sampler: bicubic_interpolation.cpp:17-30
+// (Neumann boundary, flags the tap as outside).
+__device__ __forceinline__ int neumann_clamp(int x, int n, bool& out)
+{
+    if (x < 0) { out = true; return 0; }
+    if (x >= n) { out = true; return n - 1; }
+    return x;
+}
+
+// Keys cubic convolution (a = -1/2) over four samples at fraction x:
+// bicubic_interpolation.cpp:38-46.  The literals are double in the
+// reference, the two bare differences are float, the result is rounded to
+// float.
+__device__ __forceinline__ float cubic_cell(float v0, float v1, float v2, float v3, float x)
+{
+    const double xd = (double)x;
+    const float d20 = v2 - v0;
+    const float d12 = v1 - v2;
+    const double c3 = ((3.0 * (double)d12) + (double)v3) - (double)v0;
+    const double c2 = (((2.0 * (double)v0) - (5.0 * (double)v1)) + (4.0 * (double)v2)) - (double)v3;
+    const double inner = (double)d20 + xd * (c2 + xd * c3);
+    return (float)((double)v1 + (0.5 * xd) * inner);
+}
+
+// Tap geometry of one bicubic sample: bicubic_interpolation.cpp:74-99.
+// Row `my` uses the x direction sign, as the reference does (:92).
+struct BicubicTaps {
+    int cols[4];
+    int rows[4];
+    float fx, fy;
+    bool out;
+};
+
+__device__ __forceinline__ BicubicTaps bicubic_taps(float uu, float vv, int nx, int ny)
+{
+    BicubicTaps t;
+    const int sx = (uu < 0) ? -1 : 1;
+    const int sy = (vv < 0) ? -1 : 1;
+    bool out = false;
+    const int iu = (int)uu, iv = (int)vv;
+    const int x = neumann_clamp(iu, nx, out);
+    const int y = neumann_clamp(iv, ny, out);
+    const int mx = neumann_clamp(iu - sx, nx, out);
+    const int my = neumann_clamp(iv - sx, ny, out);
+    const int dx = neumann_clamp(iu + sx, nx, out);
+    const int dy = neumann_clamp(iv + sy, ny, out);
+    const int ddx = neumann_clamp(iu + 2 * sx, nx, out);
+    const int ddy = neumann_clamp(iv + 2 * sy, ny, out);
+    t.cols[0] = mx; t.cols[1] = x; t.cols[2] = dx; t.cols[3] = ddx;
+    t.rows[0] = my; t.rows[1] = y; t.rows[2] = dy; t.rows[3] = ddy;
+    t.fx = uu - (float)x;
+    t.fy = vv - (float)y;
+    t.out = out;
+    return t;
+}
+
+// Sample `img` with precomputed taps: columns first along y, then along x
+// (bicubic_interpolation_cell, bicubic_interpolation.cpp:53-65).
+__device__ __forceinline__ float bicubic_sample(const float* __restrict__ img, int nx,
+                                                const BicubicTaps& t)
+{
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float* col = img + t.cols[c];
+        v[c] = cubic_cell(col[(size_t)nx * t.rows[0]], col[(size_t)nx * t.rows[1]],
+                          col[(size_t)nx * t.rows[2]], col[(size_t)nx * t.rows[3]], t.fy);
+    }
+    return cubic_cell(v[0], v[1], v[2], v[3], t.fx);
+}
+
+__device__ __forceinline__ float bicubic_at(const float* __restrict__ img, float uu, float vv,
+                                            int nx, int ny, bool border_out)
+{
+    const BicubicTaps t = bicubic_taps(uu, vv, nx, ny);
+    if (t.out && border_out) return 0.0f;
+    return bicubic_sample(img, nx, t);
+}
+
+// hypotf as glibc 2.35 evaluates it for finite arguments: the double sum of
+// the exact double squares, square-rooted in double, rounded to float.
+__device__ __forceinline__ float hypot_ref(float a, float b)
+{
+    return (float)__builtin_sqrt((double)a * (double)a + (double)b * (double)b);
+}
+
+}  // namespace amdflow

@@ -1,0 +1,510 @@
+// tvl1.cpp -- host orchestration of the TV-L1 solver and its C ABI.
+//
+// Mirrors TV_L1 of src/tvl1/sequential/tvl1.{hpp,cpp}: the constructor fixes
+// the pyramid, runDualTVL1Multiscale builds it and solves coarse to fine.
+// All work is enqueued on one HIP stream.  The only host involvement inside
+// a solve is pacing the iteration launches of each warping: the device
+// decides when a warping has converged and publishes its progress in a
+// host-mapped word that the host polls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "amdflow.h"
+#include "common.hpp"
+#include "tvl1_kernels.hpp"
+#include "tvl1_launch.hpp"
+
+using namespace amdflow;
+
+namespace {
+
+struct LaunchRule {
+    long long min_pixels;
+    int K;
+    int R;
+};
+
+// Built-in launch shapes per scale size (pixels): large scales are bound by
+// memory traffic and use short temporal blocking; small scales are bound by
+// launch latency and fuse more iterations per launch.
+const LaunchRule kDefaultRules[] = {
+    {200000, 2, 8},
+    {20000, 4, 4},
+    {0, 4, 4},
+};
+
+constexpr int kInFlight = 3;   // iteration launches kept queued ahead of the device
+
+}  // namespace
+
+struct of_tvl1 {
+    of_tvl1_params prm{};
+    int W = 0, H = 0, ns = 0;
+    int nx[kMaxScales]{}, ny[kMaxScales]{};
+    size_t off[kMaxScales]{};
+    size_t total = 0;
+
+    hipStream_t stream = nullptr;
+    float* I0p = nullptr;
+    float* I1p = nullptr;
+    float* ubase = nullptr;     // per scale s at 4*off[s]: [buf0 u1|u2][buf1 u1|u2]
+    float* pbase = nullptr;     // [buf0 p11|p12|p21|p22][buf1 ...], buffer stride 4*N0
+    float* I1x = nullptr;
+    float* I1y = nullptr;
+    float* wx = nullptr;
+    float* wy = nullptr;
+    float* rhoc = nullptr;
+    float* A0 = nullptr;
+    float* A1 = nullptr;
+    uint8_t* in8 = nullptr;     // 2*N0 bytes of upload staging
+    float* inf = nullptr;       // 2*N0 floats of upload staging
+    float* prev = nullptr;      // previous frame of the streaming API
+    bool have_prev = false;
+    float* dflow = nullptr;     // 2*N0
+    float* hflow = nullptr;     // pinned host copy returned by of_tvl1_flow
+    TvCtl* ctl = nullptr;
+    double* partials = nullptr;
+    int* d_iters = nullptr;
+    unsigned long long* hstatus = nullptr;   // host-mapped
+    unsigned epoch = 0;
+    std::vector<int> h_iters;
+    GaussTaps g_pre{}, g_zoom{};
+    std::vector<LaunchRule> rules;
+    long long launches_issued = 0;
+};
+
+namespace {
+
+int pick_rule(const of_tvl1* h, long long npx, int* K, int* R)
+{
+    const LaunchRule* best = nullptr;
+    for (const LaunchRule& r : h->rules)
+        if (npx >= r.min_pixels && (!best || r.min_pixels > best->min_pixels)) best = &r;
+    if (!best)
+        for (const LaunchRule& r : kDefaultRules)
+            if (npx >= r.min_pixels) { best = &r; break; }
+    if (!best) best = &kDefaultRules[2];
+    *K = best->K;
+    *R = best->R;
+    return OF_OK;
+}
+
+void release(of_tvl1* h)
+{
+    if (!h) return;
+    dev_free(h->I0p);
+    dev_free(h->I1p);
+    dev_free(h->ubase);
+    dev_free(h->pbase);
+    dev_free(h->I1x);
+    dev_free(h->I1y);
+    dev_free(h->wx);
+    dev_free(h->wy);
+    dev_free(h->rhoc);
+    dev_free(h->A0);
+    dev_free(h->A1);
+    dev_free(h->in8);
+    dev_free(h->inf);
+    dev_free(h->prev);
+    dev_free(h->dflow);
+    dev_free(h->ctl);
+    dev_free(h->partials);
+    dev_free(h->d_iters);
+    if (h->hflow) (void)hipHostFree(h->hflow);
+    if (h->hstatus) (void)hipHostFree(h->hstatus);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+// Runs the iterations of one warping: keeps up to kInFlight launches queued
+// and stops issuing once the device reports that the warping has stopped.
+int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
+{
+    const of_tvl1_params& P = h->prm;
+    if (P.max_iterations <= 0) return OF_OK;
+    int K, R;
+    pick_rule(h, (long long)h->nx[s] * h->ny[s], &K, &R);
+    const size_t N = (size_t)h->nx[s] * h->ny[s];
+    IterArgs a{};
+    a.wx = h->wx;
+    a.wy = h->wy;
+    a.rhoc = h->rhoc;
+    float* ub = h->ubase + 4 * h->off[s];
+    a.u[0] = ub;
+    a.u[1] = ub + 2 * N;
+    a.p[0] = h->pbase;
+    a.p[1] = h->pbase + 4 * (size_t)h->W * h->H;
+    a.nx = h->nx[s];
+    a.ny = h->ny[s];
+    a.plane = (int)N;
+    a.l_t = P.lambda * P.theta;
+    a.theta = P.theta;
+    a.taut = P.tau / P.theta;
+    a.eps2 = (double)(P.epsilon * P.epsilon);
+    a.max_iter = P.max_iterations;
+    a.level = s;
+    a.ctl = h->ctl;
+    a.iters_slot = h->d_iters + (size_t)s * P.warps + w;
+    a.partials = h->partials;
+    a.host_status = h->hstatus;
+    a.epoch = ++h->epoch;
+
+    const long long max_launches = (P.max_iterations + K - 1) / K + 1;
+    long long issued = 0, completed = 0;
+    bool stopped = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned spins = 0;
+    while (!stopped && issued < max_launches) {
+        while (issued < max_launches && issued - completed < kInFlight) {
+            OF_TRY(launch_iter(K, R, a, st));
+            issued++;
+            h->launches_issued++;
+        }
+        if (issued >= max_launches) break;
+        const unsigned long long word = __atomic_load_n(h->hstatus, __ATOMIC_ACQUIRE);
+        if ((unsigned)(word >> 32) == a.epoch) {
+            completed = (long long)(word & 0x7FFFFFFFull);
+            stopped = ((word >> 31) & 1ull) != 0;
+            if (stopped || issued - completed < kInFlight) continue;
+        }
+        if (++spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady) {
+                record_hip_error(q, "hipStreamQuery", __FILE__, __LINE__);
+                return OF_ERR_HIP;
+            }
+            if (q == hipSuccess) {
+                // Idle stream: every issued launch has reported.
+                const unsigned long long w2 = __atomic_load_n(h->hstatus, __ATOMIC_ACQUIRE);
+                if ((unsigned)(w2 >> 32) == a.epoch) {
+                    completed = (long long)(w2 & 0x7FFFFFFFull);
+                    stopped = ((w2 >> 31) & 1ull) != 0;
+                }
+                if (!stopped && completed < issued) return OF_ERR_STATE;
+            }
+            const double secs =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (secs > 60.0) return OF_ERR_HIP;
+        } else {
+            __builtin_ia32_pause();
+        }
+    }
+    return OF_OK;
+}
+
+// Enqueues one full multiscale solve from device frames and waits for it.
+// `is_u8` selects the element type of src0/src1.
+int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst, hipStream_t st)
+{
+    const of_tvl1_params& P = h->prm;
+    const size_t N0 = (size_t)h->W * h->H;
+    OF_TRY(launch_ctl_init(h->ctl, st));
+    if (is_u8)
+        OF_TRY(launch_load_minmax_u8((const uint8_t*)src0, (const uint8_t*)src1, h->A0, h->A1, N0,
+                                     h->ctl, st));
+    else
+        OF_TRY(launch_load_minmax_f32((const float*)src0, (const float*)src1, h->A0, h->A1, N0,
+                                      h->ctl, st));
+    float* I00 = h->I0p;
+    float* I10 = h->I1p;
+    OF_TRY(launch_normalize(h->A0, h->A1, I00, I10, N0, h->ctl, st));
+    // pre-smoothing, tvl1.cpp:185-186
+    OF_TRY(launch_gaussian(I00, h->A0, I00, h->W, h->H, h->g_pre, st));
+    OF_TRY(launch_gaussian(I10, h->A0, I10, h->W, h->H, h->g_pre, st));
+    // pyramid, tvl1.cpp:189-196 (zoom_out: blur a copy, then resample)
+    for (int s = 1; s < h->ns; s++) {
+        for (int k = 0; k < 2; k++) {
+            float* P0 = (k ? h->I1p : h->I0p);
+            OF_TRY(launch_gaussian(P0 + h->off[s - 1], h->A0, h->A1, h->nx[s - 1], h->ny[s - 1],
+                                   h->g_zoom, st));
+            OF_TRY(launch_zoom_out_sample(h->A1, P0 + h->off[s], h->nx[s - 1], h->ny[s - 1],
+                                          h->nx[s], h->ny[s], P.zfactor, st));
+        }
+    }
+    // zero flow at the coarsest scale, tvl1.cpp:199-200
+    {
+        const int s = h->ns - 1;
+        OF_TRY(launch_fill(h->ubase + 4 * h->off[s], 2 * (size_t)h->nx[s] * h->ny[s], 0.0f, st));
+    }
+    const float invZ = 1 / P.zfactor;
+    for (int s = h->ns - 1; s >= 0; s--) {
+        const int nx = h->nx[s], ny = h->ny[s];
+        const size_t N = (size_t)nx * ny;
+        float* I0s = h->I0p + h->off[s];
+        float* I1s = h->I1p + h->off[s];
+        float* ub = h->ubase + 4 * h->off[s];
+        OF_TRY(launch_centered_gradient(I1s, h->I1x, h->I1y, nx, ny, st));
+        OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:232-235
+        for (int w = 0; w < P.warps; w++) {
+            OF_TRY(launch_warp_setup(I0s, I1s, h->I1x, h->I1y, ub, ub + 2 * N, h->wx, h->wy,
+                                     h->rhoc, nx, ny, h->ctl, s, st));
+            OF_TRY(run_warping(h, s, w, st));
+        }
+        if (s > 0) {
+            float* uf = h->ubase + 4 * h->off[s - 1];
+            OF_TRY(launch_zoom_in_flow(ub, ub + 2 * N, uf, nx, ny, h->nx[s - 1], h->ny[s - 1],
+                                       invZ, h->ctl, s, st));
+        }
+    }
+    OF_TRY(launch_copy_flow(h->ubase, h->ubase + 2 * N0, dst, 2 * N0, h->ctl, st));
+    OF_HIP(hipMemcpyAsync(h->h_iters.data(), h->d_iters, sizeof(int) * h->h_iters.size(),
+                          hipMemcpyDeviceToHost, st));
+    OF_HIP(hipStreamSynchronize(st));
+    return OF_OK;
+}
+
+int validate(const of_tvl1_params& p, int W, int H)
+{
+    if (W < 2 || H < 2) return OF_ERR_ARG;
+    if ((long long)W * H >= (1ll << 31)) return OF_ERR_ARG;
+    if (!(p.tau > 0) || !(p.theta > 0) || !(p.lambda >= 0)) return OF_ERR_ARG;
+    if (!(p.zfactor > 0 && p.zfactor < 1)) return OF_ERR_ARG;
+    if (p.warps < 1 || p.nscales < 1 || p.max_iterations < 0 || !(p.epsilon >= 0)) return OF_ERR_ARG;
+    return OF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void of_tvl1_default_params(of_tvl1_params* p)
+{
+    if (!p) return;
+    p->tau = 0.25f;
+    p->lambda = 0.15f;
+    p->theta = 0.3f;
+    p->nscales = 100;
+    p->zfactor = 0.5f;
+    p->warps = 5;
+    p->epsilon = 0.01f;
+    p->max_iterations = 300;
+}
+
+int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1** out)
+{
+    if (!out) return OF_ERR_ARG;
+    *out = nullptr;
+    of_tvl1_params p;
+    if (params) p = *params;
+    else of_tvl1_default_params(&p);
+    OF_TRY(validate(p, width, height));
+
+    // scale count, tvl1.cpp:84-85, clamped to >= 1
+    const float logz = std::log(1.0f / p.zfactor);
+    const double q = std::log(std::hypot((double)width, (double)height) / 16.0) / (double)logz;
+    const float Nf = (float)(1.0 + q);
+    int ns = (int)((Nf < (float)p.nscales) ? Nf : (float)p.nscales);
+    if (ns < 1) ns = 1;
+    if (ns > kMaxScales) ns = kMaxScales;
+
+    of_tvl1* h = new (std::nothrow) of_tvl1();
+    if (!h) return OF_ERR_ALLOC;
+    h->prm = p;
+    h->W = width;
+    h->H = height;
+    h->ns = ns;
+    h->nx[0] = width;
+    h->ny[0] = height;
+    for (int s = 1; s < ns; s++) zoom_size(h->nx[s - 1], h->ny[s - 1], p.zfactor, &h->nx[s], &h->ny[s]);
+    size_t off = 0;
+    for (int s = 0; s < ns; s++) {
+        if (h->nx[s] < 2 || h->ny[s] < 2) { delete h; return OF_ERR_ARG; }
+        h->off[s] = off;
+        off += (size_t)h->nx[s] * h->ny[s];
+    }
+    h->total = off;
+
+    // Gaussian taps: pre-smoothing sigma 0.8f (tvl1.hpp:7) and the zoom_out
+    // sigma 0.6f*sqrt(1/zf^2 - 1) rounded to float (zoom.cpp:38).
+    int st = gaussian_taps((double)0.8f, &h->g_pre);
+    const float zsig = (float)((double)0.6f * std::sqrt(1.0 / (double)(p.zfactor * p.zfactor) - 1.0));
+    if (!st) st = gaussian_taps((double)zsig, &h->g_zoom);
+    if (!st && (h->g_pre.size > width || h->g_pre.size > height)) st = OF_ERR_SIGMA;
+    for (int s = 0; s + 1 < ns && !st; s++)
+        if (h->g_zoom.size > h->nx[s] || h->g_zoom.size > h->ny[s]) st = OF_ERR_SIGMA;
+    if (st) { delete h; return st; }
+
+    const size_t N0 = (size_t)width * height;
+    size_t maxpart = 1;
+    const int Ks[] = {1, 2, 4, 8}, Rs[] = {2, 4, 8};
+    for (int K : Ks)
+        for (int R : Rs)
+            if (iter_config_ok(K, R)) {
+                unsigned gx, gy;
+                iter_grid(K, R, width, height, &gx, &gy);
+                maxpart = std::max(maxpart, (size_t)gx * gy * K);
+            }
+
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->stream = nullptr;
+        release(h);
+        return OF_ERR_HIP;
+    }
+    st = OF_OK;
+    if (!st) st = dev_alloc(&h->I0p, h->total);
+    if (!st) st = dev_alloc(&h->I1p, h->total);
+    if (!st) st = dev_alloc(&h->ubase, 4 * h->total);
+    if (!st) st = dev_alloc(&h->pbase, 8 * N0);
+    if (!st) st = dev_alloc(&h->I1x, N0);
+    if (!st) st = dev_alloc(&h->I1y, N0);
+    if (!st) st = dev_alloc(&h->wx, N0);
+    if (!st) st = dev_alloc(&h->wy, N0);
+    if (!st) st = dev_alloc(&h->rhoc, N0);
+    if (!st) st = dev_alloc(&h->A0, N0);
+    if (!st) st = dev_alloc(&h->A1, N0);
+    if (!st) st = dev_alloc(&h->in8, 2 * N0);
+    if (!st) st = dev_alloc(&h->inf, 2 * N0);
+    if (!st) st = dev_alloc(&h->prev, N0);
+    if (!st) st = dev_alloc(&h->dflow, 2 * N0);
+    if (!st) st = dev_alloc(&h->ctl, 1);
+    if (!st) st = dev_alloc(&h->partials, maxpart);
+    if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
+    if (!st && hipHostMalloc((void**)&h->hflow, sizeof(float) * 2 * N0, hipHostMallocDefault) != hipSuccess) {
+        h->hflow = nullptr;
+        st = OF_ERR_ALLOC;
+    }
+    if (!st && hipHostMalloc((void**)&h->hstatus, sizeof(unsigned long long),
+                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        h->hstatus = nullptr;
+        st = OF_ERR_ALLOC;
+    }
+    if (st) { (void)hipGetLastError(); release(h); return st; }
+    *h->hstatus = 0ull;
+    std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
+    h->h_iters.assign((size_t)ns * p.warps, 0);
+    if (hipMemsetAsync(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        release(h);
+        return OF_ERR_HIP;
+    }
+    *out = h;
+    return OF_OK;
+}
+
+void of_tvl1_destroy(of_tvl1* h) { release(h); }
+
+int of_tvl1_nscales(const of_tvl1* h) { return h ? h->ns : 0; }
+
+int of_tvl1_scale_size(const of_tvl1* h, int s, int* nx, int* ny)
+{
+    if (!h || s < 0 || s >= h->ns) return OF_ERR_ARG;
+    if (nx) *nx = h->nx[s];
+    if (ny) *ny = h->ny[s];
+    return OF_OK;
+}
+
+int of_tvl1_run_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, float* dU, void* stream)
+{
+    if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    return solve(h, dI0, dI1, true, dU, st);
+}
+
+int of_tvl1_run_device_f32(of_tvl1* h, const float* dI0, const float* dI1, float* dU, void* stream)
+{
+    if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    return solve(h, dI0, dI1, false, dU, st);
+}
+
+int of_tvl1_run_u8(of_tvl1* h, const uint8_t* I0, const uint8_t* I1)
+{
+    if (!h || !I0 || !I1) return OF_ERR_ARG;
+    const size_t N0 = (size_t)h->W * h->H;
+    OF_HIP(hipMemcpyAsync(h->in8, I0, N0, hipMemcpyHostToDevice, h->stream));
+    OF_HIP(hipMemcpyAsync(h->in8 + N0, I1, N0, hipMemcpyHostToDevice, h->stream));
+    OF_TRY(solve(h, h->in8, h->in8 + N0, true, h->dflow, h->stream));
+    OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
+    OF_HIP(hipStreamSynchronize(h->stream));
+    return OF_OK;
+}
+
+int of_tvl1_run_f32(of_tvl1* h, const float* I0, const float* I1)
+{
+    if (!h || !I0 || !I1) return OF_ERR_ARG;
+    const size_t N0 = (size_t)h->W * h->H;
+    OF_HIP(hipMemcpyAsync(h->inf, I0, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+    OF_HIP(hipMemcpyAsync(h->inf + N0, I1, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+    OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
+    OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
+    OF_HIP(hipStreamSynchronize(h->stream));
+    return OF_OK;
+}
+
+int of_tvl1_push_f32(of_tvl1* h, const float* I)
+{
+    if (!h || !I) return OF_ERR_ARG;
+    const size_t N0 = (size_t)h->W * h->H;
+    if (!h->have_prev) {
+        OF_HIP(hipMemcpyAsync(h->prev, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+        OF_HIP(hipStreamSynchronize(h->stream));
+        std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
+        h->have_prev = true;
+        return OF_OK;
+    }
+    // I0 = previous frame, I1 = this frame (src/tvl1/cuda/tvl1.cu:126-130)
+    OF_HIP(hipMemcpyAsync(h->inf, h->prev, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
+    OF_HIP(hipMemcpyAsync(h->inf + N0, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+    OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
+    OF_HIP(hipMemcpyAsync(h->prev, h->inf + N0, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
+    OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
+    OF_HIP(hipStreamSynchronize(h->stream));
+    return OF_OK;
+}
+
+const float* of_tvl1_flow(of_tvl1* h) { return h ? h->hflow : nullptr; }
+
+int of_tvl1_iterations(const of_tvl1* h, int* counts, int cap)
+{
+    if (!h || !counts) return OF_ERR_ARG;
+    const int n = (int)h->h_iters.size();
+    for (int i = 0; i < n && i < cap; i++) counts[i] = h->h_iters[i];
+    return OF_OK;
+}
+
+int of_tvl1_get_stats(const of_tvl1* h, of_tvl1_stats* st)
+{
+    if (!h || !st) return OF_ERR_ARG;
+    st->nscales = h->ns;
+    st->warps = h->prm.warps;
+    long long pi = 0, sumN = 0;
+    for (int s = 0; s < h->ns; s++) {
+        const long long N = (long long)h->nx[s] * h->ny[s];
+        sumN += N;
+        for (int w = 0; w < h->prm.warps; w++) pi += (long long)h->h_iters[(size_t)s * h->prm.warps + w] * N;
+    }
+    st->pixel_iterations = pi;
+    st->pixel_warps = (long long)h->prm.warps * sumN;
+    st->pixels_all_scales = sumN;
+    st->launches = h->launches_issued;
+    const long long N0 = (long long)h->W * h->H;
+    st->algorithmic_bytes = 60.0 * (double)pi + 36.0 * (double)st->pixel_warps + 64.0 * (double)sumN +
+                            48.0 * (double)N0;
+    return OF_OK;
+}
+
+int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_launch, int rows_per_wave)
+{
+    if (!h) return OF_ERR_ARG;
+    if (min_pixels < 0) {
+        h->rules.clear();
+        return OF_OK;
+    }
+    if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
+    for (LaunchRule& r : h->rules)
+        if (r.min_pixels == min_pixels) {
+            r.K = iters_per_launch;
+            r.R = rows_per_wave;
+            return OF_OK;
+        }
+    h->rules.push_back({min_pixels, iters_per_launch, rows_per_wave});
+    return OF_OK;
+}
+
+}  // extern "C"

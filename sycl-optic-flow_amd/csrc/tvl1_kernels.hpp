@@ -1,0 +1,53 @@
+// tvl1_kernels.hpp -- device-side data structures of the TV-L1 solver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace amdflow {
+
+constexpr int kMaxScales = 32;
+
+// Device control block of one solve.  Written by the last workgroup of each
+// iteration launch (and by the set-up kernels), read by every workgroup at
+// the start of the next launch: kernel boundaries order the accesses.
+struct TvCtl {
+    int cur[kMaxScales];        // which of the two u/p buffers is current, per scale
+    int n;                      // iterations completed in the current warping
+    int stop;                   // warping finished: further launches are no-ops
+    int redo;                   // >0: re-run this many iterations from `cur`
+    unsigned counter;           // workgroups finished in the running launch
+    unsigned launches;          // iteration launches that did work (stats)
+    unsigned wlaunches;         // the same, counted within the current warping
+    unsigned long long maxkey[2];   // (|x| bits << 32) | (~index), per image
+    unsigned long long minkey[2];   // (|x| bits << 32) | index, per image
+};
+
+struct IterArgs {
+    const float* wx;
+    const float* wy;
+    const float* rhoc;
+    float* u[2];        // u[b]: u1 plane, u2 plane at +plane
+    float* p[2];        // p[b]: p11, p12, p21, p22 planes, stride `plane`
+    int nx, ny;
+    int plane;          // nx*ny
+    float l_t, theta, taut;
+    double eps2;        // (float)(epsilon*epsilon) widened
+    int max_iter;
+    int level;
+    TvCtl* ctl;
+    int* iters_slot;    // receives the warping's iteration count
+    double* partials;   // [K][nblocks] per-workgroup error partials
+    unsigned long long* host_status;  // host-mapped progress word (may be null)
+    unsigned epoch;
+    double* err_log;    // if set: per-iteration error sums, indexed by iteration
+};
+
+// Launch helpers implemented in tvl1_kernels.hip.
+int launch_iter(int K, int R, const IterArgs& a, hipStream_t s);
+bool iter_config_ok(int K, int R);
+void iter_grid(int K, int R, int nx, int ny, unsigned* gx, unsigned* gy);
+
+}  // namespace amdflow

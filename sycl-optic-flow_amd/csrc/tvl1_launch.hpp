@@ -1,0 +1,37 @@
+// tvl1_launch.hpp -- host-callable launchers of the TV-L1 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "tvl1_kernels.hpp"
+
+namespace amdflow {
+
+int launch_ctl_init(TvCtl* ctl, hipStream_t s);
+int launch_ctl_begin(TvCtl* ctl, hipStream_t s);
+int launch_load_minmax_u8(const uint8_t* a, const uint8_t* b, float* fa, float* fb, size_t n,
+                          TvCtl* ctl, hipStream_t s);
+int launch_load_minmax_f32(const float* a, const float* b, float* fa, float* fb, size_t n,
+                           TvCtl* ctl, hipStream_t s);
+int launch_normalize(const float* a, const float* b, float* oa, float* ob, size_t n,
+                     const TvCtl* ctl, hipStream_t s);
+int launch_gaussian(const float* src, float* tmp, float* dst, int nx, int ny, const GaussTaps& g,
+                    hipStream_t s);
+int launch_zoom_out_sample(const float* src, float* dst, int nx, int ny, int nxx, int nyy,
+                           float factor, hipStream_t s);
+int launch_zoom_in_flow(const float* uc0, const float* uc1, float* uf, int nx, int ny, int nxx,
+                        int nyy, float invz, const TvCtl* ctl, int coarse_level, hipStream_t s);
+int launch_centered_gradient(const float* in, float* dx, float* dy, int nx, int ny,
+                             hipStream_t s);
+int launch_fill(float* p, size_t n, float v, hipStream_t s);
+int launch_warp_setup(const float* I0, const float* I1, const float* I1x, const float* I1y,
+                      const float* ub0, const float* ub1, float* wx, float* wy, float* rhoc,
+                      int nx, int ny, TvCtl* ctl, int level, hipStream_t s);
+int launch_copy_flow(const float* ub0, const float* ub1, float* out, size_t n2, const TvCtl* ctl,
+                     hipStream_t s);
+
+int launch_hypot(const float* a, const float* b, float* o, size_t n, hipStream_t s);
+
+}  // namespace amdflow
