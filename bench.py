@@ -1,0 +1,157 @@
+"""Throughput of the TV-L1 1920x1080 solve on one MI355X.
+
+A step is one full multiscale TV-L1 solve of one synthetic 1920x1080 frame
+pair (5 scales, zfactor 0.5, 5 warpings, tau 0.25, lambda 0.15, theta 0.3,
+epsilon 0.01, at most 300 iterations per warping, zero initial flow), with
+both 8-bit frames resident in HBM before the timed region starts and the
+flow left in HBM.  Prints one JSON line.
+
+`roofline` covers the dominant kernel, the primal-dual iteration kernel:
+achieved = algorithmic bytes of the iterations its launches completed
+(60 B per pixel-iteration) / summed launch durations, both measured live by
+HIP events recorded on the solver's stream around each launch.
+`cpu_baseline` times the CPU oracle (a restatement of the reference's
+sequential C++ solver) on the same pair with OpenMP on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sycl-optic-flow_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+WORKLOAD = "tvl1_1920x1080"
+SEED = 2024
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate CPU time budget of the cpu_baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(q0, q1, budget_s: float):
+    """Times the oracle on the same pair; returns (record, flow, iterations)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # CPU checker, used here only as the measured baseline
+
+    p = O.default_params(nscales=5)
+    t0 = time.perf_counter()
+    runs = 0
+    u = it = None
+    while True:
+        u, it = O.tvl1(q0, q1, p)
+        runs += 1
+        if time.perf_counter() - t0 >= budget_s or runs >= 50:
+            break
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    rec = {"value": runs / dt, "unit": "frame-pairs/s", "cores": cores, "kind": "port",
+           "sample": f"{runs} solve(s) of the same 1920x1080 u8 pair, OpenMP, {dt:.1f} s"}
+    return rec, u, it
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(pairs are independent: N GPUs would each solve their own share)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from amdflow import TV_L1, synth
+
+    torch.cuda.set_device(0)
+    cfg = synth.CONFIGS[WORKLOAD]
+    W, H = cfg["width"], cfg["height"]
+    I0, I1, _, _ = synth.make_pair(W, H, cfg["cap"], SEED, bumps=cfg["bumps"])
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    d0 = torch.from_numpy(q0.reshape(-1)).cuda()
+    d1 = torch.from_numpy(q1.reshape(-1)).cuda()
+    dU = torch.empty(2 * W * H, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    tv = TV_L1(W, H, nscales=5)
+    for _ in range(args.warmup):
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = args.steps / dt
+    st = tv.stats()
+    iters = tv.iterations()
+
+    # roofline of the dominant kernel: per-launch HIP events, separate pass
+    tv.profile(True)
+    for _ in range(max(1, args.profile_steps)):
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
+    prof = tv.profile_results()
+    tv.profile(False)
+    dom = max(prof, key=lambda p: p["milliseconds"])
+    per_launch_bytes = dom["algorithmic_bytes"] / dom["launches"]
+    avg_launch_s = dom["milliseconds"] / dom["launches"] / 1e3
+    achieved = per_launch_bytes / avg_launch_s / 1e9
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+        "kernel": f"k_tvl1_iter<{dom['iters_per_launch']},{dom['rows_per_wave']}>",
+        "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(avg_launch_s * 1e6, 2),
+        "launches": dom["launches"],
+        "kernel_share_of_step": round(dom["milliseconds"] / max(1, args.profile_steps)
+                                      / (1e3 * dt / args.steps), 3),
+    }
+
+    flow = dU.cpu().numpy()
+    result = {
+        "metric": "TV-L1 1080p frame-pairs/s (plain vs warm-start at EPE<=0.05px); HBM GB/s",
+        "value": round(value, 3),
+        "unit": "frame-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded blurred-noise frame, sinusoid+bump motion <=20 px, noise sigma 1, u8)",
+        "config": {"workload": WORKLOAD, "width": W, "height": H, "nscales": 5, "warps": 5,
+                   "zfactor": 0.5, "tau": 0.25, "lambda": 0.15, "theta": 0.3, "epsilon": 0.01,
+                   "max_iterations": 300, "mode": "plain (zero init)", "global_batch": 1},
+        "mpixel_per_s": round(value * W * H / 1e6, 2),
+        "algorithmic_GBps_whole_solve": round(st["algorithmic_bytes"] * value / 1e9, 1),
+        "pixel_iterations_per_pair": st["pixel_iterations"],
+        "iterations_per_warping": iters.tolist(),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        rec, uo, ito = cpu_baseline(q0, q1, args.cpu_seconds)
+        result["cpu_baseline"] = rec
+        result["parity"] = {
+            "max_abs_px_vs_oracle": float(np.max(np.abs(flow - uo))),
+            "bitwise_equal": bool(np.array_equal(flow, uo)),
+            "iterations_equal": bool(np.array_equal(iters, ito)),
+        }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -111,6 +111,8 @@ typedef struct {
     long long pixels_all_scales;   /* sum of N_s                             */
     long long launches;            /* iteration-kernel launches issued       */
     double algorithmic_bytes;      /* 60*PI + 36*PW + 64*sum N_s + 48*N_0    */
+    double host_launch_seconds;    /* host time inside iteration launches    */
+    double host_wait_seconds;      /* host time waiting for device progress  */
 } of_tvl1_stats;
 int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
 
@@ -122,6 +124,42 @@ int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
  * calling with min_pixels < 0 clears all rules (back to the built-in table). */
 int of_tvl1_set_launch_rule(of_tvl1 *h, long long min_pixels, int iters_per_launch,
                             int rows_per_wave);
+
+/* Iteration launches kept queued ahead of the device within a warping
+ * (1..64, default 3).  Launches queued past convergence return at once. */
+int of_tvl1_set_inflight(of_tvl1 *h, int launches);
+
+/* Autotuner of the launch shapes (no reference counterpart): tries every
+ * supported (iters_per_launch, rows_per_wave) on all scales, times each
+ * scale with HIP events over `reps` solves of the device pair dI0/dI1
+ * (width*height bytes each) and installs the fastest shape per scale as
+ * launch rules.  `chosen` (optional) receives 2*nscales ints: the K,R pair
+ * picked for each scale, finest first; `ms_table` (optional) receives the
+ * 9 x nscales mean times in ms, shapes in the order (1,2) (2,2) (1,4) (2,4)
+ * (4,4) (1,8) (2,8) (4,8) (8,8).  Results are bit-identical for every shape;
+ * only the time changes. */
+int of_tvl1_autotune(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, int reps, int *chosen,
+                     double *ms_table);
+
+/* Per-launch timing of the primal-dual iteration kernel.  When enabled,
+ * every iteration launch is bracketed by two HIP events recorded on the
+ * stream it runs on; after each solve the elapsed times are accumulated per
+ * launch shape together with the algorithmic bytes of the iterations those
+ * launches completed (60 bytes per pixel-iteration: read u1,u2,p11,p12,p21,
+ * p22,I1wx,I1wy,rho_c, write u1,u2,p11,p12,p21,p22).  Enabling resets the
+ * accumulators.  Adds host overhead: use it outside timed regions. */
+typedef struct {
+    int iters_per_launch;
+    int rows_per_wave;
+    long long launches;            /* launches timed (including no-op ones) */
+    long long pixel_iterations;    /* iterations completed x scale pixels    */
+    double milliseconds;           /* summed launch durations                */
+    double algorithmic_bytes;      /* 60 * pixel_iterations                  */
+} of_tvl1_kernel_profile;
+int of_tvl1_profile_enable(of_tvl1 *h, int enable);
+/* Writes up to `cap` entries, one per launch shape used; *count receives
+ * the number of entries. */
+int of_tvl1_profile_get(const of_tvl1 *h, of_tvl1_kernel_profile *out, int cap, int *count);
 
 /* ------------------------------------------------------------------------
  * Horn-Schunck.  Replaces initFlow / ComputeFlow / deleteFlow_mem,

@@ -57,7 +57,16 @@ class TVL1Stats(C.Structure):
         ("nscales", C.c_int), ("warps", C.c_int),
         ("pixel_iterations", C.c_longlong), ("pixel_warps", C.c_longlong),
         ("pixels_all_scales", C.c_longlong), ("launches", C.c_longlong),
-        ("algorithmic_bytes", C.c_double),
+        ("algorithmic_bytes", C.c_double), ("host_launch_seconds", C.c_double),
+        ("host_wait_seconds", C.c_double),
+    ]
+
+
+class TVL1KernelProfile(C.Structure):
+    _fields_ = [
+        ("iters_per_launch", C.c_int), ("rows_per_wave", C.c_int),
+        ("launches", C.c_longlong), ("pixel_iterations", C.c_longlong),
+        ("milliseconds", C.c_double), ("algorithmic_bytes", C.c_double),
     ]
 
 
@@ -83,6 +92,10 @@ SIGNATURES = {
     "of_tvl1_iterations": (C.c_int, [_vp, _ip, C.c_int]),
     "of_tvl1_get_stats": (C.c_int, [_vp, C.POINTER(TVL1Stats)]),
     "of_tvl1_set_launch_rule": (C.c_int, [_vp, C.c_longlong, C.c_int, C.c_int]),
+    "of_tvl1_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, C.POINTER(C.c_double)]),
+    "of_tvl1_set_inflight": (C.c_int, [_vp, C.c_int]),
+    "of_tvl1_profile_enable": (C.c_int, [_vp, C.c_int]),
+    "of_tvl1_profile_get": (C.c_int, [_vp, C.POINTER(TVL1KernelProfile), C.c_int, _ip]),
     "of_hs_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "of_hs_destroy": (None, [_vp]),
     "of_hs_compute": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
@@ -226,6 +239,32 @@ class TV_L1:
     def set_launch_rule(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int) -> None:
         _check(_lib.of_tvl1_set_launch_rule(self._h, int(min_pixels), int(iters_per_launch),
                                             int(rows_per_wave)), "set_launch_rule")
+
+    AUTOTUNE_SHAPES = [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8), (8, 8)]
+
+    def autotune(self, d_I0: int, d_I1: int, reps: int = 2):
+        """Times every launch shape per scale on a device u8 pair and keeps the
+        fastest per scale; returns (chosen [(K,R)] per scale, ms table)."""
+        ns = self.nscales
+        chosen = np.zeros(2 * ns, np.int32)
+        table = np.zeros(len(self.AUTOTUNE_SHAPES) * ns, np.float64)
+        _check(_lib.of_tvl1_autotune(self._h, d_I0, d_I1, int(reps), chosen.ctypes.data_as(_ip),
+                                     table.ctypes.data_as(C.POINTER(C.c_double))), "autotune")
+        return ([tuple(int(v) for v in chosen[2 * s:2 * s + 2]) for s in range(ns)],
+                table.reshape(len(self.AUTOTUNE_SHAPES), ns))
+
+    def set_inflight(self, launches: int) -> None:
+        _check(_lib.of_tvl1_set_inflight(self._h, int(launches)), "set_inflight")
+
+    def profile(self, enable: bool = True) -> None:
+        _check(_lib.of_tvl1_profile_enable(self._h, int(enable)), "profile_enable")
+
+    def profile_results(self) -> list[dict]:
+        arr = (TVL1KernelProfile * 16)()
+        n = C.c_int()
+        _check(_lib.of_tvl1_profile_get(self._h, arr, 16, C.byref(n)), "profile_get")
+        return [{f: getattr(arr[i], f) for f, _ in TVL1KernelProfile._fields_}
+                for i in range(min(n.value, 16))]
 
     def clear_launch_rules(self) -> None:
         _check(_lib.of_tvl1_set_launch_rule(self._h, -1, 1, 2), "set_launch_rule")

@@ -77,6 +77,24 @@ struct of_tvl1 {
     GaussTaps g_pre{}, g_zoom{};
     std::vector<LaunchRule> rules;
     long long launches_issued = 0;
+    int inflight = kInFlight;
+    double host_launch_s = 0.0, host_wait_s = 0.0;
+
+    // per-launch event timing (of_tvl1_profile_enable)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Timed {
+        int ev;          // index of the start event; end event is ev+1
+        int K, R;
+    };
+    std::vector<Timed> timed;
+    std::vector<of_tvl1_kernel_profile> prof;
+    int scale_K[kMaxScales]{}, scale_R[kMaxScales]{};
+
+    // per-scale event timing (autotuner)
+    bool scale_timing = false;
+    std::vector<hipEvent_t> scale_ev;
+    double scale_ms[kMaxScales]{};
 };
 
 namespace {
@@ -118,8 +136,62 @@ void release(of_tvl1* h)
     dev_free(h->d_iters);
     if (h->hflow) (void)hipHostFree(h->hflow);
     if (h->hstatus) (void)hipHostFree(h->hstatus);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : h->scale_ev) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+// Launch bracketed by two events from the handle's pool.
+int timed_launch(of_tvl1* h, int K, int R, const IterArgs& a, hipStream_t st)
+{
+    const int ev = 2 * (int)h->timed.size();
+    while ((int)h->ev_pool.size() < ev + 2) {
+        hipEvent_t e;
+        OF_HIP(hipEventCreate(&e));
+        h->ev_pool.push_back(e);
+    }
+    OF_HIP(hipEventRecord(h->ev_pool[ev], st));
+    OF_TRY(launch_iter(K, R, a, st));
+    OF_HIP(hipEventRecord(h->ev_pool[ev + 1], st));
+    h->timed.push_back({ev, K, R});
+    return OF_OK;
+}
+
+of_tvl1_kernel_profile& prof_entry(of_tvl1* h, int K, int R)
+{
+    for (of_tvl1_kernel_profile& p : h->prof)
+        if (p.iters_per_launch == K && p.rows_per_wave == R) return p;
+    of_tvl1_kernel_profile p{};
+    p.iters_per_launch = K;
+    p.rows_per_wave = R;
+    h->prof.push_back(p);
+    return h->prof.back();
+}
+
+// After a synchronised solve: fold the timed launches and the completed
+// iterations into the per-shape accumulators.
+int collect_profile(of_tvl1* h)
+{
+    for (const of_tvl1::Timed& t : h->timed) {
+        float ms = 0.0f;
+        OF_HIP(hipEventElapsedTime(&ms, h->ev_pool[t.ev], h->ev_pool[t.ev + 1]));
+        of_tvl1_kernel_profile& p = prof_entry(h, t.K, t.R);
+        p.launches++;
+        p.milliseconds += ms;
+    }
+    h->timed.clear();
+    for (int s = 0; s < h->ns; s++) {
+        if (!h->scale_K[s]) continue;
+        of_tvl1_kernel_profile& p = prof_entry(h, h->scale_K[s], h->scale_R[s]);
+        const long long N = (long long)h->nx[s] * h->ny[s];
+        for (int w = 0; w < h->prm.warps; w++) {
+            const long long pi = (long long)h->h_iters[(size_t)s * h->prm.warps + w] * N;
+            p.pixel_iterations += pi;
+            p.algorithmic_bytes += 60.0 * (double)pi;
+        }
+    }
+    return OF_OK;
 }
 
 // Runs the iterations of one warping: keeps up to kInFlight launches queued
@@ -130,6 +202,8 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     if (P.max_iterations <= 0) return OF_OK;
     int K, R;
     pick_rule(h, (long long)h->nx[s] * h->ny[s], &K, &R);
+    h->scale_K[s] = K;
+    h->scale_R[s] = R;
     const size_t N = (size_t)h->nx[s] * h->ny[s];
     IterArgs a{};
     a.wx = h->wx;
@@ -158,20 +232,37 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     const long long max_launches = (P.max_iterations + K - 1) / K + 1;
     long long issued = 0, completed = 0;
     bool stopped = false;
-    const auto t0 = std::chrono::steady_clock::now();
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto tw = t0;
+    bool waiting = false;
     unsigned spins = 0;
+    const int inflight = h->inflight;
     while (!stopped && issued < max_launches) {
-        while (issued < max_launches && issued - completed < kInFlight) {
-            OF_TRY(launch_iter(K, R, a, st));
+        if (waiting && issued - completed < inflight) {
+            h->host_wait_s += std::chrono::duration<double>(clk::now() - tw).count();
+            waiting = false;
+        }
+        const auto tl = clk::now();
+        bool launched = false;
+        while (issued < max_launches && issued - completed < inflight) {
+            if (h->profiling) OF_TRY(timed_launch(h, K, R, a, st));
+            else OF_TRY(launch_iter(K, R, a, st));
             issued++;
             h->launches_issued++;
+            launched = true;
         }
+        if (launched) h->host_launch_s += std::chrono::duration<double>(clk::now() - tl).count();
         if (issued >= max_launches) break;
+        if (!waiting) {
+            waiting = true;
+            tw = clk::now();
+        }
         const unsigned long long word = __atomic_load_n(h->hstatus, __ATOMIC_ACQUIRE);
         if ((unsigned)(word >> 32) == a.epoch) {
             completed = (long long)(word & 0x7FFFFFFFull);
             stopped = ((word >> 31) & 1ull) != 0;
-            if (stopped || issued - completed < kInFlight) continue;
+            if (stopped || issued - completed < inflight) continue;
         }
         if (++spins % 4096 == 0) {
             const hipError_t q = hipStreamQuery(st);
@@ -195,6 +286,7 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
             __builtin_ia32_pause();
         }
     }
+    if (waiting) h->host_wait_s += std::chrono::duration<double>(clk::now() - tw).count();
     return OF_OK;
 }
 
@@ -239,6 +331,7 @@ int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst
         float* I0s = h->I0p + h->off[s];
         float* I1s = h->I1p + h->off[s];
         float* ub = h->ubase + 4 * h->off[s];
+        if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s], st));
         OF_TRY(launch_centered_gradient(I1s, h->I1x, h->I1y, nx, ny, st));
         OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:232-235
         for (int w = 0; w < P.warps; w++) {
@@ -251,11 +344,73 @@ int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst
             OF_TRY(launch_zoom_in_flow(ub, ub + 2 * N, uf, nx, ny, h->nx[s - 1], h->ny[s - 1],
                                        invZ, h->ctl, s, st));
         }
+        if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s + 1], st));
     }
     OF_TRY(launch_copy_flow(h->ubase, h->ubase + 2 * N0, dst, 2 * N0, h->ctl, st));
     OF_HIP(hipMemcpyAsync(h->h_iters.data(), h->d_iters, sizeof(int) * h->h_iters.size(),
                           hipMemcpyDeviceToHost, st));
     OF_HIP(hipStreamSynchronize(st));
+    if (h->profiling) OF_TRY(collect_profile(h));
+    if (h->scale_timing)
+        for (int s = 0; s < h->ns; s++) {
+            float ms = 0.0f;
+            OF_HIP(hipEventElapsedTime(&ms, h->scale_ev[2 * s], h->scale_ev[2 * s + 1]));
+            h->scale_ms[s] += ms;
+        }
+    return OF_OK;
+}
+
+// Launch-shape autotuner: every supported (iterations per launch, rows per
+// wave) pair is applied to all scales in turn; each scale's wall time on
+// the device (events around its gradient, warpings and prolongation) is
+// averaged over `reps` solves of the given pair, and each scale then keeps
+// its fastest shape.  The iteration counts, and the flow, do not depend on
+// the shape (the kernels are bit-exact for every shape), so the timings
+// compare the same work.
+int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* chosen,
+             double* table)
+{
+    static const int kShapes[][2] = {{1, 2}, {2, 2}, {1, 4}, {2, 4}, {4, 4},
+                                     {1, 8}, {2, 8}, {4, 8}, {8, 8}};
+    constexpr int nshape = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+    if (reps < 1) reps = 1;
+    if (h->scale_ev.empty()) {
+        h->scale_ev.resize(2 * kMaxScales);
+        for (hipEvent_t& e : h->scale_ev) OF_HIP(hipEventCreate(&e));
+    }
+    std::vector<double> best(h->ns, 1e30);
+    std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8);
+    const std::vector<LaunchRule> saved = h->rules;
+    for (int c = 0; c < nshape; c++) {
+        h->rules.assign(1, LaunchRule{0, kShapes[c][0], kShapes[c][1]});
+        OF_TRY(solve(h, d0, d1, true, h->dflow, h->stream));   // warm-up of this shape
+        for (int s = 0; s < h->ns; s++) h->scale_ms[s] = 0.0;
+        h->scale_timing = true;
+        int st = OF_OK;
+        for (int r = 0; r < reps && st == OF_OK; r++) st = solve(h, d0, d1, true, h->dflow, h->stream);
+        h->scale_timing = false;
+        if (st != OF_OK) {
+            h->rules = saved;
+            return st;
+        }
+        for (int s = 0; s < h->ns; s++) {
+            const double ms = h->scale_ms[s] / reps;
+            if (table) table[(size_t)c * h->ns + s] = ms;
+            if (ms < best[s]) {
+                best[s] = ms;
+                bestK[s] = kShapes[c][0];
+                bestR[s] = kShapes[c][1];
+            }
+        }
+    }
+    h->rules.clear();
+    for (int s = 0; s < h->ns; s++) {
+        h->rules.push_back(LaunchRule{(long long)h->nx[s] * h->ny[s], bestK[s], bestR[s]});
+        if (chosen) {
+            chosen[2 * s] = bestK[s];
+            chosen[2 * s + 1] = bestR[s];
+        }
+    }
     return OF_OK;
 }
 
@@ -483,9 +638,43 @@ int of_tvl1_get_stats(const of_tvl1* h, of_tvl1_stats* st)
     st->pixel_warps = (long long)h->prm.warps * sumN;
     st->pixels_all_scales = sumN;
     st->launches = h->launches_issued;
+    st->host_launch_seconds = h->host_launch_s;
+    st->host_wait_seconds = h->host_wait_s;
     const long long N0 = (long long)h->W * h->H;
     st->algorithmic_bytes = 60.0 * (double)pi + 36.0 * (double)st->pixel_warps + 64.0 * (double)sumN +
                             48.0 * (double)N0;
+    return OF_OK;
+}
+
+int of_tvl1_autotune(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, int reps, int* chosen,
+                     double* ms_table)
+{
+    if (!h || !dI0 || !dI1) return OF_ERR_ARG;
+    return autotune(h, dI0, dI1, reps, chosen, ms_table);
+}
+
+int of_tvl1_set_inflight(of_tvl1* h, int launches)
+{
+    if (!h || launches < 1 || launches > 64) return OF_ERR_ARG;
+    h->inflight = launches;
+    return OF_OK;
+}
+
+int of_tvl1_profile_enable(of_tvl1* h, int enable)
+{
+    if (!h) return OF_ERR_ARG;
+    h->profiling = enable != 0;
+    h->prof.clear();
+    h->timed.clear();
+    return OF_OK;
+}
+
+int of_tvl1_profile_get(const of_tvl1* h, of_tvl1_kernel_profile* out, int cap, int* count)
+{
+    if (!h || (!out && cap > 0)) return OF_ERR_ARG;
+    const int n = (int)h->prof.size();
+    for (int i = 0; i < n && i < cap; i++) out[i] = h->prof[i];
+    if (count) *count = n;
     return OF_OK;
 }
 

@@ -91,11 +91,24 @@ __global__ __launch_bounds__(256) void k_load_minmax(const T* __restrict__ a,
     kmax1 = wave_max_u64(kmax1);
     kmin0 = wave_min_u64(kmin0);
     kmin1 = wave_min_u64(kmin1);
+    // one atomic per key per workgroup: the grid is small (see launcher)
+    __shared__ unsigned long long sk[4][4];
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&ctl->maxkey[0], kmax0);
-        atomicMax(&ctl->maxkey[1], kmax1);
-        atomicMin(&ctl->minkey[0], kmin0);
-        atomicMin(&ctl->minkey[1], kmin1);
+        sk[0][wave] = kmax0;
+        sk[1][wave] = kmax1;
+        sk[2][wave] = kmin0;
+        sk[3][wave] = kmin1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        unsigned long long v = sk[k][0];
+        for (int w = 1; w < 4; w++) v = (k < 2) ? (sk[k][w] > v ? sk[k][w] : v) : (sk[k][w] < v ? sk[k][w] : v);
+        if (k == 0) atomicMax(&ctl->maxkey[0], v);
+        if (k == 1) atomicMax(&ctl->maxkey[1], v);
+        if (k == 2) atomicMin(&ctl->minkey[0], v);
+        if (k == 3) atomicMin(&ctl->minkey[1], v);
     }
 }
 
@@ -299,21 +312,22 @@ __global__ __launch_bounds__(256) void k_copy_flow(const float* __restrict__ ub0
 // Divergence at one pixel from the current value and the left/up neighbours
 // with the border rules of mask.cpp:44-90 (first/last column cases
 // associate left to right as written there).
+// Written with selects instead of branches; every candidate is the exact
+// expression of its reference case, so the selected value is bit-identical.
 __device__ __forceinline__ float div_rule(float v1, float v1l, float v2, float v2u, bool fr,
                                           bool lr, bool fc, bool lc)
 {
-    if ((fr || lr) && (fc || lc)) {
-        const float a = fc ? v1 : -v1l;
-        return fr ? a + v2 : a - v2u;
-    }
-    if (fr) return (v1 - v1l) + v2;
-    if (lr) return (v1 - v1l) - v2u;
-    if (fc) return (v1 + v2) - v2u;
-    if (lc) return (-v1l + v2) - v2u;
-    return (v1 - v1l) + (v2 - v2u);
+    const float a = fc ? v1 : (lc ? -v1l : (v1 - v1l));
+    const float t_first_row = a + v2;               // :70, corners :86-87
+    const float t_last_row = a - v2u;               // :71, corners :88-89
+    const float t_edge_col = (a + v2) - v2u;        // :81-82
+    const float t_body = a + (v2 - v2u);            // :59
+    return fr ? t_first_row : (lr ? t_last_row : ((fc || lc) ? t_edge_col : t_body));
 }
 
-// Thresholding operator TH, tvl1.cpp:263-287.
+// Thresholding operator TH, tvl1.cpp:263-287, branch-free: the three
+// reference cases multiply the warped gradient by l_t, -l_t or -rho/grad,
+// and the `grad < 1e-10` case stores +0 exactly as the reference does.
 __device__ __forceinline__ void th_step(float rc, float gx, float gy, float u1, float u2,
                                         float l_t, float& v1, float& v2)
 {
@@ -323,17 +337,10 @@ __device__ __forceinline__ void th_step(float rc, float gx, float gy, float u1, 
     const bool c1 = rho >= -l_t * grad;
     const bool c2 = rho > l_t * grad;
     const bool c3 = grad < 1E-10f;
-    float d1 = l_t * gx, d2 = l_t * gy;
-    if (c1) {
-        d1 = fi * gx;
-        d2 = fi * gy;
-        if (c2) {
-            d1 = -l_t * gx;
-            d2 = -l_t * gy;
-        } else if (c3) {
-            d1 = d2 = 0.0f;
-        }
-    }
+    const bool zero = c1 && !c2 && c3;
+    const float k = c1 ? (c2 ? -l_t : fi) : l_t;
+    const float d1 = zero ? 0.0f : k * gx;
+    const float d2 = zero ? 0.0f : k * gy;
     v1 = u1 + d1;
     v2 = u2 + d2;
 }
@@ -442,7 +449,8 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
                 const float nu2 = v2 + theta * div2;
                 const float e1 = nu1 - u1[r], e2 = nu2 - u2[r];
                 const float e = e1 * e1 + e2 * e2;
-                if (xout && y >= oy0 && y < oy0 + TH && y < ny) acc[t] += (double)e;
+                const bool mine = xout && y >= oy0 && y < oy0 + TH && y < ny;
+                acc[t] += mine ? (double)e : 0.0;
                 u1[r] = nu1;
                 u2[r] = nu2;
             }
@@ -502,17 +510,26 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
         if (lane == 0) sh_err[wave][t] = v;
     }
     __syncthreads();
+    // The partials are published with device-scope atomic stores, which are
+    // written through to the point of coherence of all XCDs, and the wave
+    // waits for their completion (s_waitcnt vmcnt(0)) before the counter
+    // increment; the last workgroup reads them with device-scope atomic
+    // loads.  A device-scope release fence would instead write back the
+    // XCD's whole L2 at the end of every workgroup.
     if (threadIdx.x < (unsigned)kk)
-        a.partials[(size_t)threadIdx.x * nblocks + bid] =
-            ((sh_err[0][threadIdx.x] + sh_err[1][threadIdx.x]) + sh_err[2][threadIdx.x]) +
-            sh_err[3][threadIdx.x];
+        __hip_atomic_store(a.partials + (size_t)threadIdx.x * nblocks + bid,
+                           ((sh_err[0][threadIdx.x] + sh_err[1][threadIdx.x]) +
+                            sh_err[2][threadIdx.x]) + sh_err[3][threadIdx.x],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // ---- the last workgroup reduces the partials and updates the control ----
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (threadIdx.x == 0) sh_last = atomicAdd(&ctl->counter, 1u) == nblocks - 1;
     __syncthreads();
     if (!sh_last) return;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int t = 0; t < kk; t++) {
         double s = 0.0;
         const double* col = a.partials + (size_t)t * nblocks;
@@ -560,8 +577,9 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
         const unsigned long long word = ((unsigned long long)a.epoch << 32) |
                                         ((unsigned long long)stop << 31) |
                                         (unsigned long long)(wl & 0x7FFFFFFFu);
+        // the host only reads this word itself: no ordering with other data
         if (a.host_status)
-            __hip_atomic_store(a.host_status, word, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(a.host_status, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -644,7 +662,7 @@ int launch_ctl_begin(TvCtl* ctl, hipStream_t s)
 int launch_load_minmax_u8(const uint8_t* a, const uint8_t* b, float* fa, float* fb, size_t n,
                           TvCtl* ctl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_load_minmax<uint8_t>, dim3(grid1d(n)), dim3(256), 0, s, a, b, fa, fb,
+    hipLaunchKernelGGL(k_load_minmax<uint8_t>, dim3(std::min(grid1d(n), 1024u)), dim3(256), 0, s, a, b, fa, fb,
                        (unsigned)n, ctl);
     OF_LAUNCH_CHECK("k_load_minmax<u8>");
     return OF_OK;
@@ -653,7 +671,7 @@ int launch_load_minmax_u8(const uint8_t* a, const uint8_t* b, float* fa, float* 
 int launch_load_minmax_f32(const float* a, const float* b, float* fa, float* fb, size_t n,
                            TvCtl* ctl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_load_minmax<float>, dim3(grid1d(n)), dim3(256), 0, s, a, b, fa, fb,
+    hipLaunchKernelGGL(k_load_minmax<float>, dim3(std::min(grid1d(n), 1024u)), dim3(256), 0, s, a, b, fa, fb,
                        (unsigned)n, ctl);
     OF_LAUNCH_CHECK("k_load_minmax<f32>");
     return OF_OK;

@@ -1,0 +1,290 @@
+"""Pins of the CPU oracle that do not depend on the oracle's own formulas:
+closed forms, exact identities and invariants.  Each test is chosen so that a
+plausible restatement mistake (wrong mirror, dropped border term, transposed
+operand, wrong sign) fails it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from amdflow import synth
+
+
+# ---------------------------------------------------------------- Gaussian --
+
+@pytest.mark.parametrize("sigma,size", [(float(np.float32(0.8)), 5), (1.0, 6),
+                                        (float(np.float32(0.6 * np.sqrt(3.0))), 6), (2.0, 11)])
+def test_gaussian_taps_closed_form(sigma, size):
+    B = O.gaussian_taps(sigma)
+    assert len(B) == size                                  # (int)(5 sigma) + 1 taps
+    assert abs(2 * B.sum() - B[0] - 1.0) < 1e-15          # symmetric kernel sums to 1
+    i = np.arange(size)
+    np.testing.assert_allclose(B / B[0], np.exp(-i * i / (2 * sigma * sigma)), rtol=1e-14)
+
+
+def test_gaussian_preserves_constant():
+    img = np.full((40, 50), 77.25, np.float32)
+    out = O.gaussian(img, 1.0)
+    np.testing.assert_allclose(out, 77.25, rtol=3e-7)
+
+
+def test_gaussian_impulse_response_is_tap_outer_product():
+    sigma = 1.0
+    B = O.gaussian_taps(sigma)
+    img = np.zeros((41, 43), np.float32)
+    img[20, 21] = 1.0
+    out = O.gaussian(img, sigma)
+    for dy in range(-5, 6):
+        for dx in range(-5, 6):
+            want = B[abs(dy)] * B[abs(dx)] if max(abs(dx), abs(dy)) < len(B) else 0.0
+            assert abs(out[20 + dy, 21 + dx] - want) <= 1e-7 * max(want, 1e-3)
+
+
+def test_gaussian_low_border_is_whole_sample_mirror():
+    # a line of ones at x = 1: I[-1] mirrors to I[1], so x = 0 receives 2*B1
+    sigma = 1.0
+    B = O.gaussian_taps(sigma)
+    img = np.zeros((20, 30), np.float32)
+    img[:, 1] = 1.0
+    out = O.gaussian(img, sigma)
+    np.testing.assert_allclose(out[8:12, 0], 2 * B[1], rtol=1e-6)
+    img = np.zeros((30, 20), np.float32)
+    img[1, :] = 1.0
+    out = O.gaussian(img, sigma)
+    np.testing.assert_allclose(out[0, 8:12], 2 * B[1], rtol=1e-6)
+
+
+def test_gaussian_high_border_is_half_sample_mirror():
+    # a line of ones at x = n-1: I[n] mirrors to I[n-1], so x = n-1 gets B0 + B1
+    sigma = 1.0
+    B = O.gaussian_taps(sigma)
+    img = np.zeros((20, 30), np.float32)
+    img[:, -1] = 1.0
+    out = O.gaussian(img, sigma)
+    np.testing.assert_allclose(out[8:12, -1], B[0] + B[1], rtol=1e-6)
+    img = np.zeros((30, 20), np.float32)
+    img[-1, :] = 1.0
+    out = O.gaussian(img, sigma)
+    np.testing.assert_allclose(out[-1, 8:12], B[0] + B[1], rtol=1e-6)
+
+
+def test_gaussian_rejects_window_larger_than_image():
+    with pytest.raises(ValueError):
+        O.gaussian(np.zeros((10, 4), np.float32), 1.0)    # 6 taps > width 4
+    with pytest.raises(ValueError):
+        O.gaussian(np.zeros((4, 10), np.float32), 1.0)    # 6 taps > height 4
+
+
+# ------------------------------------------------------------- bicubic/zoom --
+
+def test_bicubic_reproduces_quadratics():
+    h, w = 30, 40
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    c = [3.0, 0.5, -0.25, 0.02, -0.01, 0.015]
+    f = lambda x, y: c[0] + c[1] * x + c[2] * y + c[3] * x * x + c[4] * x * y + c[5] * y * y
+    img = f(xx, yy).astype(np.float32)
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        x, y = np.float32(rng.uniform(2, w - 3)), np.float32(rng.uniform(2, h - 3))
+        assert abs(O.bicubic_at(img, x, y, True) - f(float(x), float(y))) < 2e-4
+
+
+def test_bicubic_integer_points_return_samples():
+    rng = np.random.default_rng(3)
+    img = (rng.random((12, 14)) * 100).astype(np.float32)
+    for y in range(1, 10):
+        for x in range(1, 12):
+            assert O.bicubic_at(img, np.float32(x), np.float32(y), True) == img[y, x]
+
+
+def test_bicubic_border_out_zero_outside():
+    img = np.ones((10, 10), np.float32)
+    assert O.bicubic_at(img, np.float32(0.5), np.float32(5.0), True) == 0.0   # tap x-1 < 0
+    assert O.bicubic_at(img, np.float32(8.5), np.float32(5.0), True) == 0.0   # tap x+2 = 10 > 9
+    assert O.bicubic_at(img, np.float32(7.5), np.float32(5.0), True) == 1.0   # taps 6..9 inside
+    assert O.bicubic_at(img, np.float32(0.5), np.float32(5.0), False) == 1.0  # clamped
+
+
+def test_zoom_out_half_is_blur_then_decimation():
+    rng = np.random.default_rng(4)
+    img = (rng.random((50, 64)) * 255).astype(np.float32)
+    sig = O.zoom_sigma(0.5)
+    assert abs(sig - 0.6 * np.sqrt(3.0)) < 1e-6
+    want = O.gaussian(img, float(sig))[::2, ::2]
+    assert np.array_equal(O.zoom_out(img, 0.5), want)
+
+
+def test_zoom_in_of_constant_is_constant():
+    img = np.full((17, 23), 2.5, np.float32)
+    assert np.all(O.zoom_in(img, 46, 34) == 2.5)
+
+
+def test_pyramid_sizes_and_scale_count():
+    assert O.nscales(1920, 1080, 100, 0.5) == 8
+    assert O.nscales(1920, 1080, 5, 0.5) == 5
+    assert O.nscales(640, 480, 100, 0.5) == 6
+    assert O.nscales(16, 16, 100, 0.5) == 1
+    assert O.nscales(4, 4, 100, 0.5) == 1           # would be 0 in the reference: clamped
+    sizes = [(1920, 1080)]
+    for _ in range(4):
+        sizes.append(O.zoom_size(*sizes[-1], 0.5))
+    assert sizes == [(1920, 1080), (960, 540), (480, 270), (240, 135), (120, 68)]
+
+
+# ------------------------------------------------------ finite differences --
+
+def ramp(h, w, a, b):
+    yy, xx = np.mgrid[0:h, 0:w]
+    return (a * xx + b * yy).astype(np.float32)
+
+
+def test_centered_gradient_on_ramp():
+    f = ramp(9, 11, 3, -2)
+    dx, dy = O.centered_gradient(f)
+    assert np.all(dx[:, 1:-1] == 3) and np.all(dy[1:-1, :] == -2)
+    assert np.all(dx[:, 0] == 1.5) and np.all(dx[:, -1] == 1.5)     # one-sided x 0.5
+    assert np.all(dy[0, :] == -1) and np.all(dy[-1, :] == -1)
+
+
+def test_forward_gradient_on_ramp():
+    f = ramp(9, 11, 3, -2)
+    fx, fy = O.forward_gradient(f)
+    assert np.all(fx[:, :-1] == 3) and np.all(fx[:, -1] == 0)
+    assert np.all(fy[:-1, :] == -2) and np.all(fy[-1, :] == 0)
+
+
+@pytest.mark.parametrize("h,w", [(2, 2), (3, 5), (8, 13), (31, 7)])
+def test_divergence_is_negative_adjoint_of_forward_gradient(h, w):
+    # small integers keep every float operation exact, so equality is exact
+    rng = np.random.default_rng(h * 100 + w)
+    u = rng.integers(-8, 9, (h, w)).astype(np.float32)
+    p1 = rng.integers(-8, 9, (h, w)).astype(np.float32)
+    p2 = rng.integers(-8, 9, (h, w)).astype(np.float32)
+    fx, fy = O.forward_gradient(u)
+    d = O.divergence(p1, p2)
+    lhs = float(np.sum(fx.astype(np.float64) * p1 + fy.astype(np.float64) * p2))
+    rhs = -float(np.sum(u.astype(np.float64) * d))
+    assert lhs == rhs
+
+
+def test_divergence_border_terms_each_pinned():
+    # single-pixel dual fields: div must equal minus the adjoint's stencil
+    h, w = 5, 6
+    for y in range(h):
+        for x in range(w):
+            for comp in (0, 1):
+                p1 = np.zeros((h, w), np.float32)
+                p2 = np.zeros((h, w), np.float32)
+                (p1 if comp == 0 else p2)[y, x] = 1.0
+                d = O.divergence(p1, p2)
+                # <grad e_k, p> for every unit u = e_k
+                for yy in range(h):
+                    for xx in range(w):
+                        e = np.zeros((h, w), np.float32)
+                        e[yy, xx] = 1.0
+                        fx, fy = O.forward_gradient(e)
+                        g = (fx if comp == 0 else fy)[y, x]
+                        assert d[yy, xx] == -g
+
+
+# ---------------------------------------------------------------- TV-L1 TH --
+
+def test_th_is_the_exact_minimiser():
+    rng = np.random.default_rng(6)
+    lam, theta = 0.15, 0.3
+    l_t = np.float32(lam) * np.float32(theta)
+    n = 300
+    gx = (rng.standard_normal(n) * 20).astype(np.float32)
+    gy = (rng.standard_normal(n) * 20).astype(np.float32)
+    u1 = rng.standard_normal(n).astype(np.float32)
+    u2 = rng.standard_normal(n).astype(np.float32)
+    rc = (rng.standard_normal(n) * 30).astype(np.float32)
+    v1, v2 = O.tvl1_th(rc, gx, gy, u1, u2, l_t)
+
+    def energy(a, b, i):
+        return ((a - u1[i]) ** 2 + (b - u2[i]) ** 2) / (2 * theta) + lam * abs(
+            float(rc[i]) + float(gx[i]) * a + float(gy[i]) * b)
+
+    for i in range(n):
+        g = np.array([gx[i], gy[i]], np.float64)
+        gn = g / np.linalg.norm(g)
+        e0 = energy(float(v1[i]), float(v2[i]), i)
+        # the minimiser lies on u + s*g; search s densely and around the answer
+        s_star = np.dot([v1[i] - u1[i], v2[i] - u2[i]], gn)
+        for s in np.concatenate([np.linspace(-6, 6, 2001), s_star + np.linspace(-1e-3, 1e-3, 41)]):
+            a, b = u1[i] + s * gn[0], u2[i] + s * gn[1]
+            assert e0 <= energy(a, b, i) + 1e-4
+
+
+def test_th_zero_gradient_keeps_u():
+    z = np.zeros(4, np.float32)
+    u1 = np.array([1.0, -2.0, 0.5, 0.0], np.float32)
+    rc = np.array([3.0, -1.0, 0.0, 0.0], np.float32)
+    v1, v2 = O.tvl1_th(rc, z, z, u1, u1, np.float32(0.045))
+    assert np.all(v1 == u1) and np.all(v2 == u1)
+
+
+# ---------------------------------------------------- TV-L1 full solves --
+
+def test_tvl1_identical_frames_give_zero_flow():
+    I0, _, _, _ = synth.make_pair(96, 80, 1.0, 11)
+    q = synth.quantize(I0)
+    u, it = O.tvl1(q, q)
+    assert np.all(u == 0.0)
+    assert np.all(it == 1)   # the first iteration already meets epsilon
+
+
+def test_tvl1_constant_frames_give_zero_flow():
+    q = np.full((40, 60), 9, np.uint8)
+    u, _ = O.tvl1(q, q)
+    assert np.all(u == 0.0)
+
+
+def test_tvl1_recovers_uniform_translation():
+    h, w = 160, 200
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    f = lambda x, y: 128 + 60 * np.sin(2 * np.pi * x / 37.0) * np.cos(2 * np.pi * y / 29.0) \
+        + 40 * np.sin(2 * np.pi * (x + y) / 53.0)
+    du, dv = 1.5, -0.75
+    I0 = f(xx, yy).astype(np.float32)
+    I1 = f(xx - du, yy - dv).astype(np.float32)      # I1(x + u) = I0(x)
+    u, _ = O.tvl1(I0, I1)
+    n = h * w
+    u1, u2 = u[:n].reshape(h, w), u[n:].reshape(h, w)
+    inner = (slice(20, -20), slice(20, -20))
+    assert abs(float(np.mean(u1[inner])) - du) < 0.02
+    assert abs(float(np.mean(u2[inner])) - dv) < 0.02
+
+
+def test_tvl1_stopping_rule_bounds():
+    I0, I1, _, _ = synth.make_pair(96, 80, 3.0, 12)
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    _, it = O.tvl1(q0, q1, O.default_params(epsilon=1e3))
+    assert np.all(it == 1)
+    _, it = O.tvl1(q0, q1, O.default_params(epsilon=1e-9, max_iterations=4))
+    assert np.all(it == 4)
+
+
+def test_normalization_properties():
+    rng = np.random.default_rng(7)
+    a = (rng.random((10, 12)) * 100 + 20).astype(np.float32)
+    b = (rng.random((10, 12)) * 100 + 20).astype(np.float32)
+    na, nb = O.image_normalization(a, b)
+    lo, hi = min(a.min(), b.min()), max(a.max(), b.max())
+    assert min(na.min(), nb.min()) == 0.0 and max(na.max(), nb.max()) == 255.0
+    np.testing.assert_allclose(na, 255.0 * (a - lo) / (hi - lo), rtol=1e-6, atol=1e-4)
+    full = np.arange(256, dtype=np.float32).reshape(16, 16)
+    fa, fb = O.image_normalization(full, full)
+    assert np.array_equal(fa, full)                      # u8 full range: identity
+    c = np.full((4, 4), 3.0, np.float32)
+    ca, cb = O.image_normalization(c, c)
+    assert np.array_equal(ca, c)                         # den <= 0: unchanged
+
+
+def test_normalization_uses_abs_extremes_for_signed_input():
+    # isamax/isamin pick the largest/smallest |x|: with negative values the
+    # "max" is the value of largest magnitude (cblas semantics, tvl1.cpp:353-356)
+    a = np.array([[-10.0, 2.0, 3.0]], np.float32)
+    b = np.array([[1.0, 4.0, 5.0]], np.float32)
+    na, nb = O.image_normalization(a, b)
+    # isamax(a) -> -10, isamax(b) -> 5 : max = 5 ; isamin(a) -> 2, isamin(b) -> 1 : min = 1
+    np.testing.assert_allclose(na, 255.0 * (a - 1.0) / 4.0, rtol=1e-6)
