@@ -40,6 +40,16 @@ int of_dbg_tvl1_iterations(const float *I1wx, const float *I1wy, const float *rh
 /* hypot as the solver evaluates it on the GPU, element-wise. */
 int of_dbg_hypot(const float *a, const float *b, float *out, size_t n);
 
+/* Horn-Schunck: `iters` Jacobi sweeps (solverKernel.cuh:145-152) from
+ * (du, dv) with launch shape (sweeps_per_launch K, rows_per_wave R). */
+int of_dbg_hs_jacobi(const float *du, const float *dv, const float *Ix, const float *Iy,
+                     const float *Iz, int width, int height, float alpha, int iters, int K, int R,
+                     float *du_out, float *dv_out);
+/* Horn-Schunck: bilinear warp of I1 by (u, v), then Ix, Iy, Iz
+ * (warpingKernel.cuh, derivativesKernel.cuh intent). */
+int of_dbg_hs_warp_derivatives(const float *I0, const float *I1, const float *u, const float *v,
+                               int width, int height, float *I1w, float *Ix, float *Iy, float *Iz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -268,3 +268,7 @@ class TV_L1:
 
     def clear_launch_rules(self) -> None:
         _check(_lib.of_tvl1_set_launch_rule(self._h, -1, 1, 2), "set_launch_rule")
+
+
+from .hs import HornSchunck  # noqa: E402
+from .lk import LucasKanade, lk_pyramidal, lk_pyramidal_device  # noqa: E402

@@ -105,3 +105,28 @@ def hypot(a, b):
     out = np.empty_like(a)
     _check(_lib.of_dbg_hypot(_p(a), _p(b), _p(out), a.size), "dbg_hypot")
     return out
+
+
+_lib.of_dbg_hs_jacobi.restype = C.c_int
+_lib.of_dbg_hs_jacobi.argtypes = [_vp] * 5 + [C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                              C.c_int, _vp, _vp]
+_lib.of_dbg_hs_warp_derivatives.restype = C.c_int
+_lib.of_dbg_hs_warp_derivatives.argtypes = [_vp] * 4 + [C.c_int, C.c_int] + [_vp] * 4
+
+
+def hs_jacobi(du, dv, Ix, Iy, Iz, alpha, iters, K=4, R=4):
+    du, dv, Ix, Iy, Iz = map(_f32, (du, dv, Ix, Iy, Iz))
+    h, w = du.shape
+    o1, o2 = np.empty_like(du), np.empty_like(dv)
+    _check(_lib.of_dbg_hs_jacobi(_p(du), _p(dv), _p(Ix), _p(Iy), _p(Iz), w, h, alpha, iters, K, R,
+                                 _p(o1), _p(o2)), "dbg_hs_jacobi")
+    return o1, o2
+
+
+def hs_warp_derivatives(I0, I1, u, v):
+    I0, I1, u, v = map(_f32, (I0, I1, u, v))
+    h, w = I0.shape
+    outs = [np.empty_like(I0) for _ in range(4)]
+    _check(_lib.of_dbg_hs_warp_derivatives(_p(I0), _p(I1), _p(u), _p(v), w, h,
+                                           *[_p(o) for o in outs]), "dbg_hs_warp_derivatives")
+    return tuple(outs)  # I1w, Ix, Iy, Iz

@@ -1,0 +1,478 @@
+// lk.hip -- Lucas-Kanade on gfx950: kernels, host orchestration and C ABI.
+//
+// Single-scale LK over a ring of T u8 frames, as LucasKanade::lucas_kanade
+// (src/lk/cuda/lucaskanade.cu:54-71, kernels/kernels.cu:14-181): temporal
+// FIR It, zero-padded spatial correlation Ix, Iy of frame (iframe+1)%T,
+// w x w window sums and a Cramer 2x2 solve where det != 0.  The reference's
+// y-border test against nx (kernels.cu:106) is not reproduced and the
+// border ring it leaves unwritten is 0.
+//
+// Pyramidal iterative LK (no reference counterpart): Gaussian pyramid as in
+// TV-L1's zoom_out with the given sigma, per level the template gradient
+// (centred differences) and its structure tensor G summed over the window,
+// then `iters` x (bilinear warp of I1, b = window sums of (Ix It, Iy It),
+// u -= G^-1 b where det G > det_eps), flow carried up with zoom_in x 1/zf.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "amdflow.h"
+#include "amdflow_debug.h"
+#include "common.hpp"
+#include "tvl1_launch.hpp"
+
+namespace amdflow {
+namespace {
+
+constexpr int kMaxFilt = 9;
+
+struct Filt {
+    float f[kMaxFilt];
+};
+
+// lucaskanade.cu:79-161
+bool lk_filter(int size, bool temporal, Filt* out)
+{
+    float* f = out->f;
+    for (int i = 0; i < kMaxFilt; i++) f[i] = 0.0f;
+    switch (size) {
+    case 1: f[0] = 1.0f; return true;
+    case 2:
+        if (!temporal) return false;
+        f[0] = -1.0f; f[1] = 1.0f; return true;
+    case 3: f[0] = -1.0f / 2.0f; f[1] = 0.0f; f[2] = 1.0f / 2.0f; return true;
+    case 5:
+        f[0] = 1.0f / 12.0f; f[1] = -2.0f / 3.0f; f[2] = 0.0f; f[3] = 2.0f / 3.0f; f[4] = -1.0f / 12.0f;
+        return true;
+    case 7:
+        f[0] = -1.0f / 60.0f; f[1] = 3.0f / 20.0f; f[2] = -3.0f / 4.0f; f[3] = 0.0f;
+        f[4] = 3.0f / 4.0f; f[5] = -3.0f / 20.0f; f[6] = 1.0f / 60.0f;
+        return true;
+    case 9:
+        f[0] = 1.0f / 280.0f; f[1] = -4.0f / 105.0f; f[2] = 1.0f / 5.0f; f[3] = -4.0f / 5.0f;
+        f[4] = 0.0f; f[5] = 4.0f / 5.0f; f[6] = -1.0f / 5.0f; f[7] = 4.0f / 105.0f;
+        f[8] = -1.0f / 280.0f;
+        return true;
+    default: return false;
+    }
+}
+
+inline dim3 grid2d(int w, int h) { return dim3(cdiv(w, 64), cdiv(h, 4)); }
+
+#define LK_CHECK(name)                                      \
+    do {                                                    \
+        hipError_t e_ = hipGetLastError();                  \
+        if (e_ != hipSuccess) {                             \
+            record_hip_error(e_, name, __FILE__, __LINE__); \
+            return OF_ERR_HIP;                              \
+        }                                                   \
+    } while (0)
+
+// It, Ix, Iy of the single-scale LK (kernels.cu:14-121).
+__global__ __launch_bounds__(256) void k_lk_derivs(const uint8_t* __restrict__ ring, int T,
+                                                   int iframe, int nx, int ny, Filt ft, int spac,
+                                                   Filt fs, float* __restrict__ It,
+                                                   float* __restrict__ Ix, float* __restrict__ Iy)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= nx || i >= ny) return;
+    const size_t n = (size_t)nx * ny, p = (size_t)i * nx + j;
+    float t = 0.0f;
+    for (int k = 0; k < T; k++) t += ft.f[k] * (float)ring[(size_t)((k + iframe + 1) % T) * n + p];
+    It[p] = t;
+    const uint8_t* fr = ring + (size_t)((iframe + 1) % T) * n;
+    const int fc = (spac - 1) / 2;
+    float gx = 0.0f, gy = 0.0f;
+    for (int k = 0; k < spac; k++) {
+        const int jj = j + k - fc, ii = i + k - fc;
+        if (jj >= 0 && jj < nx) gx += (float)fr[(size_t)i * nx + jj] * fs.f[k];
+        if (ii >= 0 && ii < ny) gy += (float)fr[(size_t)ii * nx + j] * fs.f[k];
+    }
+    Ix[p] = gx;
+    Iy[p] = gy;
+}
+
+// Window sums and Cramer solve (kernels.cu:134-181); the ring of width
+// (win-1)/2 is written as 0.
+__global__ __launch_bounds__(256) void k_lk_solve_single(const float* __restrict__ Ix,
+                                                         const float* __restrict__ Iy,
+                                                         const float* __restrict__ It, int nx,
+                                                         int ny, int win, float* __restrict__ Vx,
+                                                         float* __restrict__ Vy)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= nx || i >= ny) return;
+    const size_t p = (size_t)i * nx + j;
+    const int wc = (win - 1) / 2;
+    float vx = 0.0f, vy = 0.0f;
+    if (i >= wc && j >= wc && i < ny - wc && j < nx - wc) {
+        float sxx = 0.0f, sxy = 0.0f, syy = 0.0f, sxt = 0.0f, syt = 0.0f;
+        for (int ii = -wc; ii <= wc; ii++)
+            for (int jj = -wc; jj <= wc; jj++) {
+                const size_t q = (size_t)(i + ii) * nx + (j + jj);
+                const float gx = Ix[q], gy = Iy[q], gt = It[q];
+                sxx += gx * gx;
+                sxy += gx * gy;
+                syy += gy * gy;
+                sxt += gx * gt;
+                syt += gy * gt;
+            }
+        const float det = (sxx * syy - sxy * sxy);
+        if (det != 0.0f) {
+            vx = 1.0f / det * (syy * (-sxt) + (-sxy) * (-syt));
+            vy = 1.0f / det * ((-sxy) * (-sxt) + sxx * (-syt));
+        }
+    }
+    Vx[p] = vx;
+    Vy[p] = vy;
+}
+
+// ---- pyramidal LK ----
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ float bilinear_at(const float* __restrict__ src, int w, int h, float x,
+                                             float y)
+{
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    const int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
+    const int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
+    const float a = src[(size_t)ya * w + xa], b = src[(size_t)ya * w + xb];
+    const float c = src[(size_t)yb * w + xa], d = src[(size_t)yb * w + xb];
+    const float top = (1.0f - ax) * a + ax * b;
+    const float bot = (1.0f - ax) * c + ax * d;
+    return (1.0f - ay) * top + ay * bot;
+}
+
+// Products of the template gradient: Ix^2, Ix*Iy, Iy^2 (centred differences
+// with one-sided x0.5 borders, as the TV-L1 gradient).
+__global__ __launch_bounds__(256) void k_lk_tensor_products(const float* __restrict__ P0, int w,
+                                                            int h, float* __restrict__ Ix,
+                                                            float* __restrict__ Iy,
+                                                            float* __restrict__ t3)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= w || i >= h) return;
+    const size_t p = (size_t)i * w + j, n = (size_t)w * h;
+    const int jl = j > 0 ? j - 1 : 0, jr = j < w - 1 ? j + 1 : w - 1;
+    const int iu = i > 0 ? i - 1 : 0, id = i < h - 1 ? i + 1 : h - 1;
+    const float ddx = P0[(size_t)i * w + jr] - P0[(size_t)i * w + jl];
+    const float ddy = P0[(size_t)id * w + j] - P0[(size_t)iu * w + j];
+    const float gx = (float)(0.5 * (double)ddx), gy = (float)(0.5 * (double)ddy);
+    Ix[p] = gx;
+    Iy[p] = gy;
+    t3[p] = gx * gx;
+    t3[n + p] = gx * gy;
+    t3[2 * n + p] = gy * gy;
+}
+
+// Row (dir 0) or column (dir 1) window sums of `planes` planes, window
+// clipped to the image, ascending order.
+__global__ __launch_bounds__(256) void k_lk_boxsum(const float* __restrict__ in,
+                                                   float* __restrict__ out, int w, int h,
+                                                   int planes, int wc, int dir)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= w || i >= h) return;
+    const size_t n = (size_t)w * h;
+    for (int c = 0; c < planes; c++) {
+        const float* src = in + c * n;
+        float s = 0.0f;
+        for (int k = -wc; k <= wc; k++) {
+            if (dir == 0) {
+                const int jj = j + k;
+                if (jj >= 0 && jj < w) s += src[(size_t)i * w + jj];
+            } else {
+                const int ii = i + k;
+                if (ii >= 0 && ii < h) s += src[(size_t)ii * w + j];
+            }
+        }
+        out[c * n + (size_t)i * w + j] = s;
+    }
+}
+
+// It = I1(x+u) - I0 and the two products Ix*It, Iy*It.
+__global__ __launch_bounds__(256) void k_lk_it_products(const float* __restrict__ P0,
+                                                        const float* __restrict__ P1,
+                                                        const float* __restrict__ Ix,
+                                                        const float* __restrict__ Iy,
+                                                        const float* __restrict__ u,
+                                                        const float* __restrict__ v, int w, int h,
+                                                        float* __restrict__ t2)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= w || i >= h) return;
+    const size_t p = (size_t)i * w + j, n = (size_t)w * h;
+    const float wv = bilinear_at(P1, w, h, (float)j + u[p], (float)i + v[p]);
+    const float itv = wv - P0[p];
+    t2[p] = Ix[p] * itv;
+    t2[n + p] = Iy[p] * itv;
+}
+
+// Column sums of the two row-summed products, then the 2x2 update.
+__global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ rows2,
+                                                   const float* __restrict__ G, int w, int h,
+                                                   int wc, float det_eps, float* __restrict__ u,
+                                                   float* __restrict__ v)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= w || i >= h) return;
+    const size_t p = (size_t)i * w + j, n = (size_t)w * h;
+    float bx = 0.0f, by = 0.0f;
+    for (int k = -wc; k <= wc; k++) {
+        const int ii = i + k;
+        if (ii >= 0 && ii < h) {
+            bx += rows2[(size_t)ii * w + j];
+            by += rows2[n + (size_t)ii * w + j];
+        }
+    }
+    const float gxx = G[p], gxy = G[n + p], gyy = G[2 * n + p];
+    const float det = gxx * gyy - gxy * gxy;
+    if (det > det_eps) {
+        const float du = (gxy * by - gyy * bx) / det;
+        const float dv = (gxy * bx - gxx * by) / det;
+        u[p] = u[p] + du;
+        v[p] = v[p] + dv;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lk_zero2(float* __restrict__ a, float* __restrict__ b,
+                                                  size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        a[i] = 0.0f;
+        b[i] = 0.0f;
+    }
+}
+
+unsigned g1(size_t n) { const unsigned g = cdiv(n, 256); return g > 4096 ? 4096 : (g ? g : 1); }
+
+}  // namespace
+
+// Pyramidal LK from device frames.  Workspace sized for the finest level.
+int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, float zf,
+                 float sigma, int win, int iters, float det_eps, float* du, float* dv,
+                 hipStream_t st)
+{
+    if (!dI0 || !dI1 || !du || !dv || W < 2 || H < 2) return OF_ERR_ARG;
+    if (levels < 1 || levels > 16 || win < 1 || (win & 1) == 0 || iters < 0) return OF_ERR_ARG;
+    if (!(zf > 0 && zf < 1) || !(sigma > 0)) return OF_ERR_ARG;
+    GaussTaps g;
+    OF_TRY(gaussian_taps((double)sigma, &g));
+    int nx[16], ny[16];
+    size_t off[16];
+    nx[0] = W; ny[0] = H;
+    size_t total = 0;
+    for (int l = 0; l < levels; l++) {
+        if (l) zoom_size(nx[l - 1], ny[l - 1], zf, &nx[l], &ny[l]);
+        if (nx[l] < 2 || ny[l] < 2) return OF_ERR_ARG;
+        if (l + 1 < levels && (g.size > nx[l] || g.size > ny[l])) return OF_ERR_SIGMA;
+        off[l] = total;
+        total += (size_t)nx[l] * ny[l];
+    }
+    const size_t n0 = (size_t)W * H;
+    std::vector<float*> bufs;
+    auto alloc = [&](size_t n) -> float* {
+        float* p = nullptr;
+        if (dev_alloc(&p, n) != OF_OK) return nullptr;
+        bufs.push_back(p);
+        return p;
+    };
+    float* P0 = alloc(total);
+    float* P1 = alloc(total);
+    float* A0 = alloc(n0);
+    float* A1 = alloc(n0);
+    float* Ix = alloc(n0);
+    float* Iy = alloc(n0);
+    float* PROD = alloc(3 * n0);   // per-pixel products (3 or 2 planes of the level size)
+    float* ROWS = alloc(3 * n0);   // their row sums
+    float* GT = alloc(3 * n0);     // structure tensor Gxx, Gxy, Gyy of the level
+    float* u = alloc(n0);
+    float* v = alloc(n0);
+    float* nu = alloc(n0);
+    float* nv = alloc(n0);
+    int status = OF_OK;
+    if (!P0 || !P1 || !A0 || !A1 || !Ix || !Iy || !PROD || !ROWS || !GT || !u || !v || !nu || !nv)
+        status = OF_ERR_ALLOC;
+    auto run = [&]() -> int {
+        OF_HIP(hipMemcpyAsync(P0, dI0, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        OF_HIP(hipMemcpyAsync(P1, dI1, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        for (int l = 1; l < levels; l++) {
+            OF_TRY(launch_gaussian(P0 + off[l - 1], A0, A1, nx[l - 1], ny[l - 1], g, st));
+            OF_TRY(launch_zoom_out_sample(A1, P0 + off[l], nx[l - 1], ny[l - 1], nx[l], ny[l], zf, st));
+            OF_TRY(launch_gaussian(P1 + off[l - 1], A0, A1, nx[l - 1], ny[l - 1], g, st));
+            OF_TRY(launch_zoom_out_sample(A1, P1 + off[l], nx[l - 1], ny[l - 1], nx[l], ny[l], zf, st));
+        }
+        const int wc = (win - 1) / 2;
+        const float invZ = 1 / zf;
+        {
+            const size_t nc = (size_t)nx[levels - 1] * ny[levels - 1];
+            hipLaunchKernelGGL(k_lk_zero2, dim3(g1(nc)), dim3(256), 0, st, u, v, nc);
+            LK_CHECK("k_lk_zero2");
+        }
+        for (int l = levels - 1; l >= 0; l--) {
+            const int w = nx[l], h = ny[l];
+            const float* p0 = P0 + off[l];
+            const float* p1 = P1 + off[l];
+            hipLaunchKernelGGL(k_lk_tensor_products, grid2d(w, h), dim3(256), 0, st, p0, w, h, Ix, Iy,
+                               PROD);
+            LK_CHECK("k_lk_tensor_products");
+            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 3, wc, 0);
+            LK_CHECK("k_lk_boxsum");
+            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, 3, wc, 1);
+            LK_CHECK("k_lk_boxsum");
+            for (int it = 0; it < iters; it++) {
+                hipLaunchKernelGGL(k_lk_it_products, grid2d(w, h), dim3(256), 0, st, p0, p1, Ix, Iy,
+                                   u, v, w, h, PROD);
+                LK_CHECK("k_lk_it_products");
+                hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 2, wc, 0);
+                LK_CHECK("k_lk_boxsum");
+                hipLaunchKernelGGL(k_lk_update, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, wc,
+                                   det_eps, u, v);
+                LK_CHECK("k_lk_update");
+            }
+            if (l > 0) {
+                // zoom_in (zoom.cpp:61-85) of both components, times 1/zf
+                OF_TRY(launch_zoom_in_planes(u, v, nu, nv, w, h, nx[l - 1], ny[l - 1], invZ, st));
+                float* t = u; u = nu; nu = t;
+                t = v; v = nv; nv = t;
+            }
+        }
+        OF_HIP(hipMemcpyAsync(du, u, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        OF_HIP(hipMemcpyAsync(dv, v, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        OF_HIP(hipStreamSynchronize(st));
+        return OF_OK;
+    };
+    if (status == OF_OK) status = run();
+    for (float* p : bufs) dev_free(p);
+    return status;
+}
+
+}  // namespace amdflow
+
+using namespace amdflow;
+
+struct of_lk {
+    int spac = 3, temp = 2, win = 5, nx = 0, ny = 0;
+    Filt fs{}, ft{};
+    hipStream_t stream = nullptr;
+    uint8_t* ring = nullptr;
+    float *It = nullptr, *Ix = nullptr, *Iy = nullptr, *Vx = nullptr, *Vy = nullptr;
+};
+
+namespace {
+void lk_release(of_lk* h)
+{
+    if (!h) return;
+    dev_free(h->ring);
+    dev_free(h->It); dev_free(h->Ix); dev_free(h->Iy); dev_free(h->Vx); dev_free(h->Vy);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+}  // namespace
+
+extern "C" {
+
+int of_lk_create(int spac_filt_size, int temp_filt_size, int window_size, int nx, int ny, of_lk** out)
+{
+    if (!out) return OF_ERR_ARG;
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || window_size < 1 || (window_size & 1) == 0) return OF_ERR_ARG;
+    if ((long long)nx * ny >= (1ll << 31)) return OF_ERR_ARG;
+    Filt fs, ft;
+    if (!lk_filter(spac_filt_size, false, &fs) || !lk_filter(temp_filt_size, true, &ft))
+        return OF_ERR_UNSUPPORTED;
+    of_lk* h = new (std::nothrow) of_lk();
+    if (!h) return OF_ERR_ALLOC;
+    h->spac = spac_filt_size;
+    h->temp = temp_filt_size;
+    h->win = window_size;
+    h->nx = nx;
+    h->ny = ny;
+    h->fs = fs;
+    h->ft = ft;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->stream = nullptr;
+        lk_release(h);
+        return OF_ERR_HIP;
+    }
+    const size_t n = (size_t)nx * ny;
+    int st = dev_alloc(&h->ring, (size_t)temp_filt_size * n);
+    float** bufs[] = {&h->It, &h->Ix, &h->Iy, &h->Vx, &h->Vy};
+    for (float** b : bufs)
+        if (!st) st = dev_alloc(b, n);
+    if (!st && hipMemsetAsync(h->ring, 0, (size_t)temp_filt_size * n, h->stream) != hipSuccess) st = OF_ERR_HIP;
+    if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = OF_ERR_HIP;
+    if (st) { lk_release(h); return st; }
+    *out = h;
+    return OF_OK;
+}
+
+void of_lk_destroy(of_lk* h) { lk_release(h); }
+
+int of_lk_push_u8(of_lk* h, const uint8_t* frame, int temp_size, int fr, int frame_size)
+{
+    if (!h || !frame || temp_size != h->temp || fr < 0) return OF_ERR_ARG;
+    if ((size_t)frame_size != (size_t)h->nx * h->ny) return OF_ERR_ARG;
+    const int idx = fr % temp_size;
+    OF_HIP(hipMemcpyAsync(h->ring + (size_t)idx * frame_size, frame, (size_t)frame_size,
+                          hipMemcpyHostToDevice, h->stream));
+    OF_HIP(hipStreamSynchronize(h->stream));
+    return OF_OK;
+}
+
+int of_lk_compute(of_lk* h, float* Vx, float* Vy, int iframe)
+{
+    if (!h || !Vx || !Vy || iframe < 0) return OF_ERR_ARG;
+    const size_t n = (size_t)h->nx * h->ny;
+    hipLaunchKernelGGL(k_lk_derivs, grid2d(h->nx, h->ny), dim3(256), 0, h->stream, h->ring, h->temp,
+                       iframe, h->nx, h->ny, h->ft, h->spac, h->fs, h->It, h->Ix, h->Iy);
+    LK_CHECK("k_lk_derivs");
+    hipLaunchKernelGGL(k_lk_solve_single, grid2d(h->nx, h->ny), dim3(256), 0, h->stream, h->Ix,
+                       h->Iy, h->It, h->nx, h->ny, h->win, h->Vx, h->Vy);
+    LK_CHECK("k_lk_solve_single");
+    OF_HIP(hipMemcpyAsync(Vx, h->Vx, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    OF_HIP(hipMemcpyAsync(Vy, h->Vy, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    OF_HIP(hipStreamSynchronize(h->stream));
+    return OF_OK;
+}
+
+int of_lk_pyr_compute_device(const float* dI0, const float* dI1, int width, int height, int levels,
+                             float zf, float sigma, int window, int iters, float det_eps, float* du,
+                             float* dv, void* stream)
+{
+    return lk_pyr_solve(dI0, dI1, width, height, levels, zf, sigma, window, iters, det_eps, du, dv,
+                        (hipStream_t)stream);
+}
+
+int of_lk_pyr_compute(const float* I0, const float* I1, int width, int height, int levels, float zf,
+                      float sigma, int window, int iters, float det_eps, float* u, float* v)
+{
+    if (!I0 || !I1 || !u || !v || width < 2 || height < 2) return OF_ERR_ARG;
+    const size_t n = (size_t)width * height;
+    float *d0 = nullptr, *d1 = nullptr, *du = nullptr, *dv = nullptr;
+    int st = dev_alloc(&d0, n);
+    if (!st) st = dev_alloc(&d1, n);
+    if (!st) st = dev_alloc(&du, n);
+    if (!st) st = dev_alloc(&dv, n);
+    if (!st && hipMemcpy(d0, I0, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
+    if (!st && hipMemcpy(d1, I1, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
+    if (!st) st = lk_pyr_solve(d0, d1, width, height, levels, zf, sigma, window, iters, det_eps, du, dv, nullptr);
+    if (!st && hipMemcpy(u, du, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) st = OF_ERR_HIP;
+    if (!st && hipMemcpy(v, dv, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) st = OF_ERR_HIP;
+    dev_free(d0); dev_free(d1); dev_free(du); dev_free(dv);
+    return st;
+}
+
+}  // extern "C"

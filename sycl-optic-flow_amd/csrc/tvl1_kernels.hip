@@ -223,6 +223,24 @@ __global__ __launch_bounds__(256) void k_zoom_in_flow(const float* __restrict__ 
     uf[nf + (size_t)i1 * nxx + j1] = invz * b;
 }
 
+// zoom_in of two separate planes times `scale` (used by the pyramidal LK).
+__global__ __launch_bounds__(256) void k_zoom_in_planes(const float* __restrict__ a,
+                                                        const float* __restrict__ b,
+                                                        float* __restrict__ oa,
+                                                        float* __restrict__ ob, int nx, int ny,
+                                                        int nxx, int nyy, float factorx,
+                                                        float factory, float scale)
+{
+    const int j1 = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i1 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j1 >= nxx || i1 >= nyy) return;
+    const float i2 = (float)i1 / factory;
+    const float j2 = (float)j1 / factorx;
+    const BicubicTaps t = bicubic_taps(j2, i2, nx, ny);
+    oa[(size_t)i1 * nxx + j1] = scale * bicubic_sample(a, nx, t);
+    ob[(size_t)i1 * nxx + j1] = scale * bicubic_sample(b, nx, t);
+}
+
 // ---------------------------------------------------------------------------
 // centred gradient (mask.cpp:152-209)
 // ---------------------------------------------------------------------------
@@ -713,6 +731,17 @@ int launch_zoom_in_flow(const float* uc0, const float* uc1, float* uf, int nx, i
     hipLaunchKernelGGL(k_zoom_in_flow, grid2d(nxx, nyy), dim3(256), 0, s, uc0, uc1, uf, nx, ny,
                        nxx, nyy, factorx, factory, invz, ctl, coarse_level);
     OF_LAUNCH_CHECK("k_zoom_in_flow");
+    return OF_OK;
+}
+
+int launch_zoom_in_planes(const float* a, const float* b, float* oa, float* ob, int nx, int ny,
+                          int nxx, int nyy, float scale, hipStream_t s)
+{
+    const float factorx = ((float)nxx / (float)nx);
+    const float factory = ((float)nyy / (float)ny);
+    hipLaunchKernelGGL(k_zoom_in_planes, grid2d(nxx, nyy), dim3(256), 0, s, a, b, oa, ob, nx, ny,
+                       nxx, nyy, factorx, factory, scale);
+    OF_LAUNCH_CHECK("k_zoom_in_planes");
     return OF_OK;
 }
 

@@ -23,6 +23,8 @@ int launch_zoom_out_sample(const float* src, float* dst, int nx, int ny, int nxx
                            float factor, hipStream_t s);
 int launch_zoom_in_flow(const float* uc0, const float* uc1, float* uf, int nx, int ny, int nxx,
                         int nyy, float invz, const TvCtl* ctl, int coarse_level, hipStream_t s);
+int launch_zoom_in_planes(const float* a, const float* b, float* oa, float* ob, int nx, int ny,
+                          int nxx, int nyy, float scale, hipStream_t s);
 int launch_centered_gradient(const float* in, float* dx, float* dy, int nx, int ny,
                              hipStream_t s);
 int launch_fill(float* p, size_t n, float v, hipStream_t s);

@@ -1,0 +1,109 @@
+"""GPU parity of the Horn-Schunck and Lucas-Kanade paths against the CPU
+oracle: every stage and every full solve must match bit for bit (the kernels
+use the oracle's operation order, no FMA contraction, IEEE division)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from amdflow import HornSchunck, LucasKanade, lk_pyramidal, synth
+from amdflow import debug as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(rng, h, w, scale=1.0):
+    return (rng.standard_normal((h, w)) * scale).astype(np.float32)
+
+
+@pytest.mark.parametrize("K,R", [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8),
+                                 (8, 8)])
+@pytest.mark.parametrize("h,w,iters", [(37, 70, 5), (130, 257, 13), (480, 640, 3)])
+def test_hs_jacobi_bitexact(K, R, h, w, iters):
+    rng = np.random.default_rng(K * 10 + R + h)
+    du, dv = _rand(rng, h, w, 0.2), _rand(rng, h, w, 0.2)
+    Ix, Iy, Iz = _rand(rng, h, w, 5.0), _rand(rng, h, w, 5.0), _rand(rng, h, w, 3.0)
+    gu, gv = D.hs_jacobi(du, dv, Ix, Iy, Iz, 15.0, iters, K, R)
+    ou, ov = du, dv
+    for _ in range(iters):
+        ou, ov = O.hs_jacobi(ou, ov, Ix, Iy, Iz, 15.0)
+    assert np.array_equal(gu, ou) and np.array_equal(gv, ov)
+
+
+@pytest.mark.parametrize("scale", [0.0, 0.7, 4.0])
+def test_hs_warp_derivatives_bitexact(scale):
+    rng = np.random.default_rng(int(scale * 10) + 1)
+    h, w = 61, 95
+    I0 = (rng.random((h, w)) * 255).astype(np.float32)
+    I1 = (rng.random((h, w)) * 255).astype(np.float32)
+    u, v = _rand(rng, h, w, scale), _rand(rng, h, w, scale)
+    I1w, Ix, Iy, Iz = D.hs_warp_derivatives(I0, I1, u, v)
+    ow = np.empty_like(I0)
+    O._lib.orc_hs_warp(O._p(I1), w, h, w, O._p(u), O._p(v), O._p(ow))
+    ox, oy, oz = O.hs_derivatives(I0, ow)
+    assert np.array_equal(I1w, ow)
+    if scale == 0.0:
+        assert np.array_equal(I1w, I1)          # zero flow warps to the identity
+    for g, o in zip((Ix, Iy, Iz), (ox, oy, oz)):
+        assert np.array_equal(g, o)
+
+
+def test_hs_640x480_single_scale_matches_oracle():
+    """The HS workload: 640x480, |flow| <= 2 px, alpha 15, 100 Jacobi sweeps."""
+    cfg = synth.CONFIGS["hs_640x480"]
+    I0, I1, fu, fv = synth.make_pair(cfg["width"], cfg["height"], cfg["cap"], 7)
+    w, h = cfg["width"], cfg["height"]
+    with HornSchunck(1, w, w, h) as hs:
+        u, v = hs.ComputeFlow(I0, I1, w, h, w, 15.0, 1, 1, 100)
+    ou, ov = O.hs_compute(I0, I1, alpha=15.0, nLevels=1, nWarpIters=1, nSolverIters=100)
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+
+
+@pytest.mark.parametrize("shape", [(1, 8), (4, 4), (8, 8)])
+def test_hs_multilevel_strided_matches_oracle(shape):
+    w, h = 300, 200
+    stride = 320
+    I0, I1, _, _ = synth.make_pair(w, h, 4.0, 8)
+    P0 = np.zeros((h, stride), np.float32); P0[:, :w] = I0
+    P1 = np.zeros((h, stride), np.float32); P1[:, :w] = I1
+    with HornSchunck(3, stride, w, h) as hs:
+        hs.set_jacobi_shape(*shape)
+        u, v = hs.ComputeFlow(P0, P1, w, h, stride, 0.2, 3, 2, 23)
+    ou, ov = O.hs_compute(P0[:, :w], P1[:, :w], alpha=0.2, nLevels=3, nWarpIters=2,
+                          nSolverIters=23, stride=stride)
+    assert np.array_equal(u[:, :w], ou[:, :w]) and np.array_equal(v[:, :w], ov[:, :w])
+
+
+@pytest.mark.parametrize("spac,temp,win", [(3, 2, 5), (5, 2, 7), (7, 3, 3), (9, 5, 9), (1, 2, 5)])
+def test_lk_single_scale_matches_oracle(spac, temp, win):
+    rng = np.random.default_rng(spac * 10 + temp)
+    h, w = 90, 130
+    frames = [synth.quantize(synth.make_pair(w, h, 1.0, 40 + k)[0]) for k in range(temp + 2)]
+    Vx = np.zeros(h * w, np.float32)
+    Vy = np.zeros(h * w, np.float32)
+    ring = np.zeros((temp, h, w), np.uint8)
+    with LucasKanade(spac, temp, win, w, h) as lk:
+        for fr, f in enumerate(frames):
+            lk.copy_frames_circularbuffer_GPU_wrapper(f, temp, fr, w * h)
+            ring[fr % temp] = f
+            if fr >= temp - 1:
+                lk.lucas_kanade(Vx, Vy, fr)
+                ox, oy = O.lk_single(ring, fr, spac, win)
+                assert np.array_equal(Vx.reshape(h, w), ox)
+                assert np.array_equal(Vy.reshape(h, w), oy)
+
+
+def test_lk_pyramidal_1280x720_matches_oracle():
+    """The LK workload: 1280x720, |flow| <= 8 px, 3 levels, 7x7, 5 iterations/level."""
+    cfg = synth.CONFIGS["lk_1280x720"]
+    I0, I1, _, _ = synth.make_pair(cfg["width"], cfg["height"], cfg["cap"], 21)
+    u, v = lk_pyramidal(I0, I1, levels=3, zf=0.5, sigma=1.0, window=7, iters=5, det_eps=1e-6)
+    ou, ov = O.lk_pyr(I0, I1, levels=3, zf=0.5, sigma=1.0, win=7, iters=5, det_eps=1e-6)
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+
+
+@pytest.mark.parametrize("levels,win,iters", [(1, 5, 3), (2, 9, 4), (4, 7, 2)])
+def test_lk_pyramidal_variants_match_oracle(levels, win, iters):
+    I0, I1, _, _ = synth.make_pair(257, 190, 3.0, 22 + levels)
+    u, v = lk_pyramidal(I0, I1, levels=levels, window=win, iters=iters)
+    ou, ov = O.lk_pyr(I0, I1, levels=levels, win=win, iters=iters)
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)

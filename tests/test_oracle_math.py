@@ -288,3 +288,93 @@ def test_normalization_uses_abs_extremes_for_signed_input():
     na, nb = O.image_normalization(a, b)
     # isamax(a) -> -10, isamax(b) -> 5 : max = 5 ; isamin(a) -> 2, isamin(b) -> 1 : min = 1
     np.testing.assert_allclose(na, 255.0 * (a - 1.0) / 4.0, rtol=1e-6)
+
+
+# ----------------------------------------------------------- Horn-Schunck --
+
+def test_hs_jacobi_uniform_field_closed_form():
+    # uniform Ix, Iy, Iz and du = 0: each sweep maps s = Ix du + Iy dv to
+    # (alpha s - |g|^2 Iz)/(|g|^2 + alpha), so du_N = -(Ix Iz/|g|^2)(1 - r^N),
+    # r = alpha/(|g|^2 + alpha), at pixels farther than N+1 from the border.
+    h, w, N = 64, 64, 20
+    Ix, Iy, Iz, alpha = 3.0, -2.0, 1.5, 15.0
+    g2 = Ix * Ix + Iy * Iy
+    r = alpha / (g2 + alpha)
+    A = lambda v: np.full((h, w), v, np.float32)
+    du, dv = A(0.0), A(0.0)
+    for _ in range(N):
+        du, dv = O.hs_jacobi(du, dv, A(Ix), A(Iy), A(Iz), alpha)
+    want_u = -(Ix * Iz / g2) * (1 - r ** N)
+    want_v = -(Iy * Iz / g2) * (1 - r ** N)
+    np.testing.assert_allclose(du, want_u, rtol=2e-5)   # uniform field: borders agree too
+    np.testing.assert_allclose(dv, want_v, rtol=2e-5)
+
+
+def test_hs_derivatives_of_ramp_and_identity_warp():
+    h, w = 20, 24
+    I0 = ramp(h, w, 3, -2)
+    I1 = I0 + np.float32(4.0)
+    z = np.zeros((h, w), np.float32)
+    I1w = np.empty_like(I1)
+    O._lib.orc_hs_warp(O._p(I1), w, h, w, O._p(z), O._p(z), O._p(I1w))
+    assert np.array_equal(I1w, I1)                       # zero flow: identity
+    Ix, Iy, Iz = O.hs_derivatives(I0, I1w)
+    assert np.all(Ix[:, 2:-2] == 3) and np.all(Iy[2:-2, :] == -2) and np.all(Iz == 4)
+
+
+def test_hs_recovers_small_translation():
+    h, w = 96, 128
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    f = lambda x, y: 128 + 50 * np.sin(2 * np.pi * x / 41.0) + 40 * np.cos(2 * np.pi * y / 33.0)
+    I0 = f(xx, yy).astype(np.float32)
+    I1 = f(xx - 0.4, yy - 0.25).astype(np.float32)
+    u, v = O.hs_compute(I0, I1, alpha=15.0, nLevels=1, nWarpIters=3, nSolverIters=200)
+    inner = (slice(24, -24), slice(24, -24))
+    assert abs(float(np.median(u[inner])) - 0.4) < 0.05
+    assert abs(float(np.median(v[inner])) - 0.25) < 0.05
+
+
+# ----------------------------------------------------------- Lucas-Kanade --
+
+def test_lk_single_matches_least_squares():
+    # independent formulation: per window, lstsq of [Ix Iy] d = -It
+    rng = np.random.default_rng(9)
+    h, w, win, spac = 14, 17, 5, 3
+    frames = rng.integers(0, 256, (2, h, w)).astype(np.uint8)
+    vx, vy = O.lk_single(frames, 1, spac, win)
+    f0, f1 = frames[0].astype(np.float64), frames[1].astype(np.float64)
+    It = f1 - f0                                         # temporal [-1, 1]: frame1 - frame0
+    fr = f0                                              # spatial filters on frame (iframe+1)%2 = 0
+    Ix = np.zeros_like(fr); Iy = np.zeros_like(fr)
+    Ix[:, 1:-1] = 0.5 * (fr[:, 2:] - fr[:, :-2]); Ix[:, 0] = 0.5 * fr[:, 1]; Ix[:, -1] = -0.5 * fr[:, -2]
+    Iy[1:-1, :] = 0.5 * (fr[2:, :] - fr[:-2, :]); Iy[0, :] = 0.5 * fr[1, :]; Iy[-1, :] = -0.5 * fr[-2, :]
+    wc = win // 2
+    for i in range(wc, h - wc):
+        for j in range(wc, w - wc):
+            A = np.stack([Ix[i - wc:i + wc + 1, j - wc:j + wc + 1].ravel(),
+                          Iy[i - wc:i + wc + 1, j - wc:j + wc + 1].ravel()], 1)
+            b = -It[i - wc:i + wc + 1, j - wc:j + wc + 1].ravel()
+            d = np.linalg.lstsq(A, b, rcond=None)[0]
+            assert abs(vx[i, j] - d[0]) <= 2e-3 * max(1.0, abs(d[0]))
+            assert abs(vy[i, j] - d[1]) <= 2e-3 * max(1.0, abs(d[1]))
+    assert np.all(vx[:wc, :] == 0) and np.all(vx[:, -wc:] == 0)   # border ring is 0
+
+
+def test_lk_det_threshold_branch_on_ramp():
+    # a pure x-ramp has Iy = 0 everywhere: det G = 0, no update at all
+    I0 = ramp(40, 50, 2, 0)
+    u, v = O.lk_pyr(I0, I0 + 1, levels=1, win=7, iters=3)
+    assert np.all(u == 0) and np.all(v == 0)
+
+
+def test_lk_pyramidal_recovers_translation():
+    h, w = 180, 240
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    f = lambda x, y: 128 + 50 * np.sin(2 * np.pi * x / 47.0) * np.cos(2 * np.pi * y / 39.0) \
+        + 30 * np.sin(2 * np.pi * (x - y) / 61.0)
+    I0 = f(xx, yy).astype(np.float32)
+    I1 = f(xx - 3.2, yy + 1.7).astype(np.float32)
+    u, v = O.lk_pyr(I0, I1, levels=3, win=7, iters=5)
+    inner = (slice(30, -30), slice(30, -30))
+    assert abs(float(np.median(u[inner])) - 3.2) < 0.05
+    assert abs(float(np.median(v[inner])) + 1.7) < 0.05
