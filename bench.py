@@ -39,7 +39,23 @@ def parse():
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--retune", action="store_true", help="re-run the launch-shape autotuner")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for FETCH/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 1 GiB, three times: under rocprofv3
+    --pmc its FETCH_SIZE / WRITE_SIZE per dispatch give the counters' scale
+    (expected 1 GiB each)."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "dispatches": 3}), flush=True)
 
 
 def cpu_baseline(q0, q1, budget_s: float):
@@ -69,6 +85,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(pairs are independent: N GPUs would each solve their own share)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import numpy as np
     import torch

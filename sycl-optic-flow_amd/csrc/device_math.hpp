@@ -87,6 +87,21 @@ __device__ __forceinline__ float bicubic_at(const float* __restrict__ img, float
     return bicubic_sample(img, nx, t);
 }
 
+// Whole-wave lane shifts through DPP (no LDS traffic).  from_left(v) gives
+// lane i the value of lane i-1 and from_right(v) the value of lane i+1; the
+// lane without a source keeps its own value.
+__device__ __forceinline__ float from_left(float v)
+{
+    const int i = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(i, i, 0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float from_right(float v)
+{
+    const int i = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(i, i, 0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+
 // hypotf as glibc 2.35 evaluates it for finite arguments: the double sum of
 // the exact double squares, square-rooted in double, rounded to float.
 __device__ __forceinline__ float hypot_ref(float a, float b)

@@ -15,6 +15,7 @@
 #include "amdflow.h"
 #include "amdflow_debug.h"
 #include "common.hpp"
+#include "device_math.hpp"
 
 namespace amdflow {
 namespace {
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(256) void k_hs_jacobi(const float* __restrict__ du0
     __shared__ float sh_first[2][2][NW][64];   // [buf][u/v][wave][lane]: row 0 of each wave
     __shared__ float sh_last[2][2][NW][64];    // row R-1 of each wave
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int x = (int)blockIdx.x * TW - K + lane;
     const int ybase = (int)blockIdx.y * TH - K + wave * R;
     const int oy0 = (int)blockIdx.y * TH;
@@ -179,8 +181,8 @@ __global__ __launch_bounds__(256) void k_hs_jacobi(const float* __restrict__ du0
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int y = ybase + r;
-            const float lu = __shfl_up(u[r], 1, 64), lv = __shfl_up(v[r], 1, 64);
-            const float ru = __shfl_down(u[r], 1, 64), rv = __shfl_down(v[r], 1, 64);
+            const float lu = from_left(u[r]), lv = from_left(v[r]);
+            const float ru = from_right(u[r]), rv = from_right(v[r]);
             const float left_u = fc ? u[r] : lu, left_v = fc ? v[r] : lv;
             const float right_u = lc ? u[r] : ru, right_v = lc ? v[r] : rv;
             // "up" is row y+1 and "down" row y-1, clamped at the image edge

@@ -403,7 +403,8 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
     if (kk > K) kk = K;
     if (kk <= 0) return;
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int nx = a.nx, ny = a.ny;
     const int x = (int)blockIdx.x * TW - K + lane;
     const int ybase = (int)blockIdx.y * TH - K + wave * R;
@@ -411,6 +412,9 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
     const bool xin = x >= 0 && x < nx;
     const bool xout = xin && lane >= K && lane < K + TW;
     const bool fc = x == 0, lc = x == nx - 1;
+    // waves holding no first/last image column take the body formula of the
+    // divergence on every row away from the first/last image row
+    const bool edge_cols = __any(fc || lc);
 
     const float* uin = a.u[cur];
     const float* pin = a.p[cur];
@@ -454,13 +458,19 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int y = ybase + r;
-                const float p11l = __shfl_up(p11[r], 1, 64);
-                const float p21l = __shfl_up(p21[r], 1, 64);
+                const float p11l = from_left(p11[r]);
+                const float p21l = from_left(p21[r]);
                 const float p12u = r ? p12[r - 1] : up12;
                 const float p22u = r ? p22[r - 1] : up22;
                 const bool fr = y == 0, lr = y == ny - 1;
-                const float div1 = div_rule(p11[r], p11l, p12[r], p12u, fr, lr, fc, lc);
-                const float div2 = div_rule(p21[r], p21l, p22[r], p22u, fr, lr, fc, lc);
+                float div1, div2;
+                if (!edge_cols && !fr && !lr) {   // uniform branch: interior rows of interior waves
+                    div1 = (p11[r] - p11l) + (p12[r] - p12u);
+                    div2 = (p21[r] - p21l) + (p22[r] - p22u);
+                } else {
+                    div1 = div_rule(p11[r], p11l, p12[r], p12u, fr, lr, fc, lc);
+                    div2 = div_rule(p21[r], p21l, p22[r], p22u, fr, lr, fc, lc);
+                }
                 float v1, v2;
                 th_step(rc[r], gx[r], gy[r], u1[r], u2[r], l_t, v1, v2);
                 const float nu1 = v1 + theta * div1;
@@ -481,8 +491,8 @@ __global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int y = ybase + r;
-                const float u1r = __shfl_down(u1[r], 1, 64);
-                const float u2r = __shfl_down(u2[r], 1, 64);
+                const float u1r = from_right(u1[r]);
+                const float u2r = from_right(u2[r]);
                 const float u1d = r < R - 1 ? u1[r + 1] : dn1;
                 const float u2d = r < R - 1 ? u2[r + 1] : dn2;
                 const bool xl = x < nx - 1, yl = y < ny - 1;
