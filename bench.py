@@ -39,6 +39,9 @@ def parse():
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--retune", action="store_true", help="re-run the launch-shape autotuner")
+    ap.add_argument("--batch-pairs", type=int, default=64,
+                    help="pairs of the batched mixed-size measurement (0 = skip)")
+    ap.add_argument("--batch-streams", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -56,6 +59,52 @@ def calibrate_copy():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": 4 * n, "dispatches": 3}), flush=True)
+
+
+BATCH_SIZES = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
+
+
+def batched(npairs: int, streams: int, reps: int):
+    """64 pairs whose sizes are drawn uniformly (seeded) from BATCH_SIZES,
+    same generator and parameters as the 1920x1080 workload, solved by the
+    batched solver; two generated pairs per size are cycled through."""
+    import numpy as np
+    import torch
+
+    from amdflow import synth
+    from amdflow.batch import TVL1Batch
+
+    rng = np.random.default_rng(SEED + 1)
+    pick = rng.integers(0, len(BATCH_SIZES), npairs)
+    cache = {}
+    d0, d1, dU, ws, hs = [], [], [], [], []
+    for k, s in enumerate(pick):
+        w, h = BATCH_SIZES[int(s)]
+        key = (w, h, k % 2)
+        if key not in cache:
+            I0, I1, _, _ = synth.make_pair(w, h, 20.0, SEED + 7 * len(cache), bumps=True)
+            cache[key] = (torch.from_numpy(synth.quantize(I0).reshape(-1)).cuda(),
+                          torch.from_numpy(synth.quantize(I1).reshape(-1)).cuda())
+        a, b = cache[key]
+        d0.append(a.data_ptr()); d1.append(b.data_ptr())
+        dU.append(torch.empty(2 * w * h, dtype=torch.float32, device="cuda"))
+        ws.append(w); hs.append(h)
+    up = [t.data_ptr() for t in dU]
+    with TVL1Batch(streams=streams, nscales=5) as b:
+        b.run_device_u8(ws, hs, d0, d1, up)                 # warm-up (handles, caches)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            its = b.run_device_u8(ws, hs, d0, d1, up)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+    mpx = sum(w * h for w, h in zip(ws, hs)) / 1e6
+    return {"workload": f"tvl1_batch{npairs}_mixed", "pairs": npairs, "streams": streams,
+            "sizes": [f"{w}x{h}" for w, h in BATCH_SIZES],
+            "pair_counts": [int((pick == i).sum()) for i in range(len(BATCH_SIZES))],
+            "ms_per_batch": round(dt * 1e3, 2), "pairs_per_s": round(npairs / dt, 2),
+            "mpixel_per_s": round(mpx / dt, 1), "iterations_total": int(np.sum(its)),
+            "data": "synthetic, 2 generated pairs per size cycled through the batch"}
 
 
 def cpu_baseline(q0, q1, budget_s: float):
@@ -177,6 +226,8 @@ def main():
         "launch_shapes_source": "autotuned in this run" if tuned_now else "tuning table",
         "roofline": roofline,
     }
+    if args.batch_pairs > 0:
+        result["batched"] = batched(args.batch_pairs, args.batch_streams, reps=3)
     if not args.no_cpu:
         rec, uo, ito = cpu_baseline(q0, q1, args.cpu_seconds)
         result["cpu_baseline"] = rec

@@ -120,24 +120,29 @@ int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
  * least `min_pixels`: `iters_per_launch` primal-dual iterations run in one
  * launch on tiles of 64 x (4*rows_per_wave) pixels with a halo of
  * iters_per_launch pixels.  Supported: iters_per_launch in {1,2,4,8},
- * rows_per_wave in {2,4,8}.  Rules are matched largest min_pixels first;
- * calling with min_pixels < 0 clears all rules (back to the built-in table). */
+ * rows_per_wave in {2,4,8}.  `persistent` = 1 runs each warping of those
+ * scales as one cooperative launch (chunks of iters_per_launch iterations
+ * separated by grid barriers) instead of host-paced launches.  Rules are
+ * matched largest min_pixels first; calling with min_pixels < 0 clears all
+ * rules (back to the built-in table). */
 int of_tvl1_set_launch_rule(of_tvl1 *h, long long min_pixels, int iters_per_launch,
-                            int rows_per_wave);
+                            int rows_per_wave, int persistent);
 
 /* Iteration launches kept queued ahead of the device within a warping
  * (1..64, default 3).  Launches queued past convergence return at once. */
 int of_tvl1_set_inflight(of_tvl1 *h, int launches);
 
 /* Autotuner of the launch shapes (no reference counterpart): tries every
- * supported (iters_per_launch, rows_per_wave) on all scales, times each
- * scale with HIP events over `reps` solves of the device pair dI0/dI1
- * (width*height bytes each) and installs the fastest shape per scale as
- * launch rules.  `chosen` (optional) receives 2*nscales ints: the K,R pair
- * picked for each scale, finest first; `ms_table` (optional) receives the
- * 9 x nscales mean times in ms, shapes in the order (1,2) (2,2) (1,4) (2,4)
- * (4,4) (1,8) (2,8) (4,8) (8,8).  Results are bit-identical for every shape;
- * only the time changes. */
+ * supported (iters_per_launch, rows_per_wave) in both launch modes
+ * (host-paced, persistent) on all scales, times each scale with HIP events
+ * over `reps` solves of the device pair dI0/dI1 (width*height bytes each)
+ * and installs the fastest variant per scale as launch rules.  `chosen`
+ * (optional) receives 3*nscales ints: K, R, persistent for each scale,
+ * finest first; `ms_table` (optional) receives 18 x nscales mean times in
+ * ms: rows 0-8 host-paced, 9-17 persistent, shapes in the order (1,2) (2,2)
+ * (1,4) (2,4) (4,4) (1,8) (2,8) (4,8) (8,8); -1 marks a variant the device
+ * cannot run.  Results are bit-identical for every variant; only the time
+ * changes. */
 int of_tvl1_autotune(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, int reps, int *chosen,
                      double *ms_table);
 
@@ -151,6 +156,7 @@ int of_tvl1_autotune(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, int rep
 typedef struct {
     int iters_per_launch;
     int rows_per_wave;
+    int persistent;                /* 1: whole-warping cooperative kernel    */
     long long launches;            /* launches timed (including no-op ones) */
     long long pixel_iterations;    /* iterations completed x scale pixels    */
     double milliseconds;           /* summed launch durations                */
@@ -160,6 +166,110 @@ int of_tvl1_profile_enable(of_tvl1 *h, int enable);
 /* Writes up to `cap` entries, one per launch shape used; *count receives
  * the number of entries. */
 int of_tvl1_profile_get(const of_tvl1 *h, of_tvl1_kernel_profile *out, int cap, int *count);
+
+/* ------------------------------------------------------------------------
+ * Batched TV-L1 (no reference counterpart): many frame pairs of mixed sizes
+ * solved concurrently by `streams` host workers, each with its own HIP
+ * stream and one solver handle per frame size, taking pairs longest first.
+ * Results are identical to solving each pair alone.
+ * ---------------------------------------------------------------------- */
+typedef struct of_tvl1_batch of_tvl1_batch;
+int of_tvl1_batch_create(const of_tvl1_params *params, int streams, of_tvl1_batch **out);
+void of_tvl1_batch_destroy(of_tvl1_batch *b);
+/* Launch shape for scales of at least `min_pixels` pixels in this batch's
+ * handles (host-paced launches). */
+int of_tvl1_batch_set_shape(of_tvl1_batch *b, long long min_pixels, int iters_per_launch,
+                            int rows_per_wave);
+/* Solves pair i: device u8 frames dI0[i], dI1[i] of widths[i] x heights[i],
+ * flow (planar [u1|u2]) into device buffer dU[i].  `iterations_total`
+ * (optional) receives each pair's total primal-dual iteration count.
+ * Returns when all pairs are done. */
+int of_tvl1_batch_run_u8(of_tvl1_batch *b, int npairs, const int *widths, const int *heights,
+                         const uint8_t *const *dI0, const uint8_t *const *dI1, float *const *dU,
+                         int *iterations_total);
+
+/* ------------------------------------------------------------------------
+ * Two-phase TV-L1 solve (no reference counterpart; used by warm starts).
+ * of_tvl1_run_device_u8 = prepare + solve_prepared(NULL, 0, 0).
+ * ---------------------------------------------------------------------- */
+/* Phase 1 of runDualTVL1Multiscale (src/tvl1/sequential/tvl1.cpp:139-155):
+ * frames to float, joint normalisation, pre-smoothing, both pyramids. */
+int of_tvl1_prepare_device_u8(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, void *stream);
+/* Device pointers to pyramid scale s of the prepared I0 and I1 (row-major,
+ * of_tvl1_scale_size), valid until the next prepare or run.  OF_ERR_STATE
+ * before the first prepare. */
+int of_tvl1_scale_images(const of_tvl1 *h, int s, const float **dI0s, const float **dI1s);
+/* Phase 2 (tvl1.cpp:157-172): coarse-to-fine solve from `dUinit`, a device
+ * planar [u1|u2] flow at the coarsest scale in that scale's pixel units, or
+ * zero flow when NULL (as the reference).  At most `warps` warpings per
+ * scale (<= the handle's) and `max_iterations` per warping; 0 selects the
+ * handle's value.  Flow into dU (device, 2*width*height floats). */
+int of_tvl1_solve_prepared(of_tvl1 *h, const float *dUinit, int warps, int max_iterations,
+                           float *dU, void *stream);
+/* Copies the flow that scale s ended with in the last solve (planar, that
+ * scale's size and units) into dUs. */
+int of_tvl1_scale_flow(of_tvl1 *h, int s, float *dUs, void *stream);
+/* TV-L1 energy of the device flow dUs at scale s of the prepared pair:
+ * E = lambda * sum |I1(x + u) - I0(x)| + sum (|grad u1| + |grad u2|), with
+ * I1 sampled by the bicubic of the warps (clamped border) and forward
+ * differences.  energy[0] = E, energy[1] = data sum, energy[2] = TV sum. */
+int of_tvl1_energy(of_tvl1 *h, int s, const float *dUs, double *energy, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart).  A conv3x3 network,
+ * 5 -> 32 channels, `hidden_layers` x (32 -> 32), 32 -> 2, ReLU between,
+ * predicts the coarsest-scale flow from that scale's I0, I1, Ix, Iy, It.
+ * Convolutions run as implicit GEMMs on bf16 MFMA with fp32 accumulation;
+ * tensors are NHWC fp32 in device memory.  Parameters (of_cnn_get_params
+ * order): for each layer W[out][tap 0..8][in] then bias[out].
+ * ---------------------------------------------------------------------- */
+typedef struct of_cnn of_cnn;
+/* channels must be 32; He-normal initial weights from `seed`. */
+int of_cnn_create(int hidden_layers, int channels, unsigned long long seed, of_cnn **out);
+void of_cnn_destroy(of_cnn *h);
+long long of_cnn_num_params(const of_cnn *h);
+int of_cnn_get_params(const of_cnn *h, float *host);
+/* Loads fp32 parameters and resets the optimiser state. */
+int of_cnn_set_params(of_cnn *h, const float *host);
+/* Network input of one scale: dX[H*W][5] = (I0, I1, Ix, Iy, It) / 255 with
+ * centred differences of I1 (clamped at the border). */
+int of_cnn_features(const float *dI0s, const float *dI1s, int H, int W, float *dX, void *stream);
+/* dX: [B][H][W][5] -> dY: [B][H][W][2].  Asynchronous on `stream`. */
+int of_cnn_forward(of_cnn *h, const float *dX, int B, int H, int W, float *dY, void *stream);
+/* Features + forward of one scale pair; dU receives planar [u1|u2]. */
+int of_cnn_predict(of_cnn *h, const float *dI0s, const float *dI1s, int H, int W, float *dU,
+                   void *stream);
+/* Loss L = sum (Y - T)^2 / (2*B*H*W) against planar targets dT[B][2][H*W];
+ * host_grads (optional) receives dL/dparams, loss (optional) L. */
+int of_cnn_gradients(of_cnn *h, const float *dX, const float *dT, int B, int H, int W,
+                     float *host_grads, double *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss. */
+int of_cnn_train_step(of_cnn *h, const float *dX, const float *dT, int B, int H, int W, float lr,
+                      double *loss);
+/* Warm-started TV-L1 of one device u8 pair: predicts the coarsest flow,
+ * keeps it if its energy (of_tvl1_energy) is not above zero flow's, and
+ * solves with at most `warps` warpings and `max_iterations` per warping;
+ * otherwise (*fell_back = 1) runs the plain solve with the handle's
+ * parameters.  energies (optional): [0] predicted, [1] zero flow. */
+int of_tvl1_warm_run_device_u8(of_tvl1 *tv, of_cnn *net, const uint8_t *dI0, const uint8_t *dI1,
+                               int warps, int max_iterations, float *dU, int *fell_back,
+                               double *energies, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Synthetic frame pairs on the device (no reference counterpart; the
+ * measurement and training inputs).  frame0 = U(0,1) noise, Gaussian blur
+ * sigma 2, rescaled to [0,255]; flow = sum of 4 sinusoids per component
+ * plus `nbumps` Gaussian bumps, scaled to max |flow| = cap; frame1 =
+ * frame0 sampled bilinearly (clamped) at x - flow(x) plus N(0, noise^2);
+ * both rounded and clamped to u8.  params: for each component k and
+ * sinusoid j, (period, angle, phase, amplitude) at params[(4k + j)*4], then
+ * nbumps x (cx, cy, radius, du, dv) from params[32].  dflow receives the
+ * scaled flow (planar), dscratch needs 2*width*height + 4 floats.
+ * Asynchronous on `stream`.
+ * ---------------------------------------------------------------------- */
+int of_synth_pair(int width, int height, float cap, float noise, unsigned long long seed,
+                  const float *params, int nbumps, uint8_t *dI0, uint8_t *dI1, float *dflow,
+                  float *dscratch, void *stream);
 
 /* ------------------------------------------------------------------------
  * Horn-Schunck.  Replaces initFlow / ComputeFlow / deleteFlow_mem,

@@ -49,6 +49,10 @@ int of_dbg_hs_jacobi(const float *du, const float *dv, const float *Ix, const fl
  * (warpingKernel.cuh, derivativesKernel.cuh intent). */
 int of_dbg_hs_warp_derivatives(const float *I0, const float *I1, const float *u, const float *v,
                                int width, int height, float *I1w, float *Ix, float *Iy, float *Iz);
+/* Warm-start network: D = A * B for 16x16 row-major fp32 matrices with
+ * bf16-rounded operands, through the convolution kernel's MFMA path (a
+ * 1x16 image of 16 channels, centre tap only). */
+int of_dbg_cnn_mfma(const float *A, const float *B, float *D);
 
 #ifdef __cplusplus
 }

@@ -533,8 +533,12 @@ void orc_tvl1_default_params(orc_tvl1_params *p)
  * (tvl1.cpp:160-165 converts u8 with a plain cast).  `u` receives planar
  * [u1 | u2] (2*W*H floats).  `iters` receives nscales*warps counts, finest
  * scale first.  Returns the number of scales used via *nscales_out. */
-int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height,
-                     const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
+/* The multiscale driver (tvl1.cpp:139-175).  `uinit` (planar [u1|u2] at the
+ * coarsest scale) replaces the zero initial flow of tvl1.cpp:157-158 when
+ * not NULL: the warm-start extension, which has no reference counterpart. */
+static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
+                    const orc_tvl1_params *prm, const float *uinit, float *u, int *iters,
+                    int *nscales_out)
 {
     const int ns = orc_tvl1_nscales(width, height, prm->nscales, prm->zfactor);
     if (nscales_out) *nscales_out = ns;
@@ -564,7 +568,10 @@ int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height
     }
     if (!st) {
         const size_t nc = (size_t)nx[ns - 1] * ny[ns - 1];
-        for (size_t i = 0; i < nc; i++) u1s[ns - 1][i] = u2s[ns - 1][i] = 0.0f;
+        for (size_t i = 0; i < nc; i++) {
+            u1s[ns - 1][i] = uinit ? uinit[i] : 0.0f;
+            u2s[ns - 1][i] = uinit ? uinit[nc + i] : 0.0f;
+        }
         const float invZ = 1 / prm->zfactor;
         for (int s = ns - 1; s > 0 && !st; s--) {
             st = orc_dual_tvl1(I0s[s], I1s[s], u1s[s], u2s[s], nx[s], ny[s], prm,
@@ -588,9 +595,16 @@ int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height
     return st;
 }
 
-/* tvl1.cpp:160-165 then the multiscale driver. */
-int orc_tvl1_run_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
-                    const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
+int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height,
+                     const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
+{
+    return tvl1_run(I0in, I1in, width, height, prm, NULL, u, iters, nscales_out);
+}
+
+/* tvl1.cpp:160-165 then the multiscale driver, from `uinit` (or zero). */
+int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
+                         const orc_tvl1_params *prm, const float *uinit, float *u, int *iters,
+                         int *nscales_out)
 {
     const size_t size = (size_t)width * height;
     float *f0 = (float *)malloc(sizeof(float) * size);
@@ -600,10 +614,40 @@ int orc_tvl1_run_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
         f0[i] = (float)I0[i];
         f1[i] = (float)I1[i];
     }
-    const int st = orc_tvl1_run_f32(f0, f1, width, height, prm, u, iters, nscales_out);
+    const int st = tvl1_run(f0, f1, width, height, prm, uinit, u, iters, nscales_out);
     free(f0);
     free(f1);
     return st;
+}
+
+int orc_tvl1_run_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
+                    const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
+{
+    return orc_tvl1_run_init_u8(I0, I1, width, height, prm, NULL, u, iters, nscales_out);
+}
+
+/* TV-L1 energy of a flow (no reference counterpart; the warm start's
+ * fallback test): lambda * sum |I1(x + u) - I0| + sum (|grad u1| + |grad u2|),
+ * I1 sampled by the bicubic of bicubic_interpolation.cpp:108-133 with the
+ * clamped border, forward differences as mask.cpp:86-131, |.| = hypotf. */
+double orc_tvl1_energy(const float *I0, const float *I1, const float *u1, const float *u2, int nx,
+                       int ny, float lambda, double *data_out, double *tv_out)
+{
+    double data = 0.0, tv = 0.0;
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            const size_t i = (size_t)y * nx + x;
+            const float w = orc_bicubic_at(I1, (float)x + u1[i], (float)y + u2[i], nx, ny, 0);
+            data += (double)fabsf(w - I0[i]);
+            const float ax = x < nx - 1 ? u1[i + 1] - u1[i] : 0.0f;
+            const float ay = y < ny - 1 ? u1[i + nx] - u1[i] : 0.0f;
+            const float bx = x < nx - 1 ? u2[i + 1] - u2[i] : 0.0f;
+            const float by = y < ny - 1 ? u2[i + nx] - u2[i] : 0.0f;
+            tv += (double)hypotf(ax, ay) + (double)hypotf(bx, by);
+        }
+    if (data_out) *data_out = data;
+    if (tv_out) *tv_out = tv;
+    return (double)lambda * data + tv;
 }
 
 /* ------------------------------------------------------------------------- */

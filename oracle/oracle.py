@@ -70,6 +70,10 @@ _sig("orc_tvl1_iteration", C.c_double, [_vp] * 9 + [C.c_int, C.c_int, C.c_float,
 _sig("orc_dual_tvl1", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _ip])
 _sig("orc_tvl1_run_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _ip, _ip])
 _sig("orc_tvl1_run_f32", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _ip, _ip])
+_sig("orc_tvl1_run_init_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _vp, _ip,
+                                       _ip])
+_sig("orc_tvl1_energy", C.c_double, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)])
 _sig("orc_bilinear_at", C.c_float, [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float])
 _sig("orc_hs_downscale", None, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int])
 _sig("orc_hs_upscale", None, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
@@ -256,6 +260,37 @@ def tvl1(I0, I1, params: Params | None = None):
     if st:
         raise ValueError(f"tvl1: status {st}")
     return u, iters.reshape(ns, p.warps)
+
+
+def tvl1_init(I0, I1, uinit, params: Params | None = None):
+    """Multiscale TV-L1 of a uint8 pair from the initial coarsest-scale flow
+    `uinit` (planar [u1|u2] at that scale; None = zero, the reference's
+    start).  Returns (u, iteration counts [nscales, warps])."""
+    p = params or default_params()
+    I0 = np.ascontiguousarray(I0, dtype=np.uint8)
+    h, w = I0.shape
+    ns = nscales(w, h, p.nscales, p.zfactor)
+    u = np.zeros(2 * w * h, np.float32)
+    iters = np.zeros(ns * p.warps, np.int32)
+    nso = C.c_int()
+    b = np.ascontiguousarray(I1, dtype=np.uint8)
+    ui = None if uinit is None else _f32(uinit)
+    st = _lib.orc_tvl1_run_init_u8(_p(I0), _p(b), w, h, C.byref(p), None if ui is None else _p(ui),
+                                   _p(u), iters.ctypes.data_as(_ip), C.byref(nso))
+    if st:
+        raise ValueError(f"tvl1_init: status {st}")
+    return u, iters.reshape(ns, p.warps)
+
+
+def tvl1_energy(I0, I1, u, lambda_=0.15):
+    """(E, data sum, TV sum) of the planar flow u on the scale images I0, I1."""
+    a, b = _f32(I0), _f32(I1)
+    h, w = a.shape
+    uu = _f32(u)
+    d, t = C.c_double(), C.c_double()
+    e = _lib.orc_tvl1_energy(_p(a), _p(b), _p(uu), _p(uu[h * w:]), w, h, lambda_, C.byref(d),
+                             C.byref(t))
+    return e, d.value, t.value
 
 
 def bilinear_at(img, x, y) -> float:

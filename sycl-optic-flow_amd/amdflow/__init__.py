@@ -64,7 +64,7 @@ class TVL1Stats(C.Structure):
 
 class TVL1KernelProfile(C.Structure):
     _fields_ = [
-        ("iters_per_launch", C.c_int), ("rows_per_wave", C.c_int),
+        ("iters_per_launch", C.c_int), ("rows_per_wave", C.c_int), ("persistent", C.c_int),
         ("launches", C.c_longlong), ("pixel_iterations", C.c_longlong),
         ("milliseconds", C.c_double), ("algorithmic_bytes", C.c_double),
     ]
@@ -91,11 +91,16 @@ SIGNATURES = {
     "of_tvl1_run_device_f32": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "of_tvl1_iterations": (C.c_int, [_vp, _ip, C.c_int]),
     "of_tvl1_get_stats": (C.c_int, [_vp, C.POINTER(TVL1Stats)]),
-    "of_tvl1_set_launch_rule": (C.c_int, [_vp, C.c_longlong, C.c_int, C.c_int]),
+    "of_tvl1_set_launch_rule": (C.c_int, [_vp, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "of_tvl1_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, C.POINTER(C.c_double)]),
     "of_tvl1_set_inflight": (C.c_int, [_vp, C.c_int]),
     "of_tvl1_profile_enable": (C.c_int, [_vp, C.c_int]),
     "of_tvl1_profile_get": (C.c_int, [_vp, C.POINTER(TVL1KernelProfile), C.c_int, _ip]),
+    "of_tvl1_prepare_device_u8": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "of_tvl1_scale_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
+    "of_tvl1_solve_prepared": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp]),
+    "of_tvl1_scale_flow": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "of_tvl1_energy": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(C.c_double), _vp]),
     "of_hs_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "of_hs_destroy": (None, [_vp]),
     "of_hs_compute": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
@@ -236,22 +241,26 @@ class TV_L1:
         _check(_lib.of_tvl1_get_stats(self._h, C.byref(st)), "stats")
         return {f: getattr(st, f) for f, _ in TVL1Stats._fields_}
 
-    def set_launch_rule(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int) -> None:
+    def set_launch_rule(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int,
+                        persistent: bool = False) -> None:
         _check(_lib.of_tvl1_set_launch_rule(self._h, int(min_pixels), int(iters_per_launch),
-                                            int(rows_per_wave)), "set_launch_rule")
+                                            int(rows_per_wave), int(bool(persistent))),
+               "set_launch_rule")
 
     AUTOTUNE_SHAPES = [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8), (8, 8)]
+    AUTOTUNE_VARIANTS = [(k, r, 0) for k, r in AUTOTUNE_SHAPES] + [(k, r, 1) for k, r in AUTOTUNE_SHAPES]
 
     def autotune(self, d_I0: int, d_I1: int, reps: int = 2):
-        """Times every launch shape per scale on a device u8 pair and keeps the
-        fastest per scale; returns (chosen [(K,R)] per scale, ms table)."""
+        """Times every launch variant (K, R, persistent) per scale on a device u8
+        pair and keeps the fastest per scale; returns (chosen [(K, R, P)] per
+        scale, ms table [18 x nscales])."""
         ns = self.nscales
-        chosen = np.zeros(2 * ns, np.int32)
-        table = np.zeros(len(self.AUTOTUNE_SHAPES) * ns, np.float64)
+        chosen = np.zeros(3 * ns, np.int32)
+        table = np.zeros(len(self.AUTOTUNE_VARIANTS) * ns, np.float64)
         _check(_lib.of_tvl1_autotune(self._h, d_I0, d_I1, int(reps), chosen.ctypes.data_as(_ip),
                                      table.ctypes.data_as(C.POINTER(C.c_double))), "autotune")
-        return ([tuple(int(v) for v in chosen[2 * s:2 * s + 2]) for s in range(ns)],
-                table.reshape(len(self.AUTOTUNE_SHAPES), ns))
+        return ([tuple(int(v) for v in chosen[3 * s:3 * s + 3]) for s in range(ns)],
+                table.reshape(len(self.AUTOTUNE_VARIANTS), ns))
 
     def set_inflight(self, launches: int) -> None:
         _check(_lib.of_tvl1_set_inflight(self._h, int(launches)), "set_inflight")
@@ -267,7 +276,31 @@ class TV_L1:
                 for i in range(min(n.value, 16))]
 
     def clear_launch_rules(self) -> None:
-        _check(_lib.of_tvl1_set_launch_rule(self._h, -1, 1, 2), "set_launch_rule")
+        _check(_lib.of_tvl1_set_launch_rule(self._h, -1, 1, 2, 0), "set_launch_rule")
+
+    # two-phase solve (warm starts)
+    def prepare_device_u8(self, d_I0: int, d_I1: int, stream: int = 0) -> None:
+        _check(_lib.of_tvl1_prepare_device_u8(self._h, d_I0, d_I1, stream or None), "prepare")
+
+    def scale_images(self, s: int) -> tuple[int, int]:
+        """Device pointers to the prepared pyramid images of scale s."""
+        a, b = _vp(), _vp()
+        _check(_lib.of_tvl1_scale_images(self._h, int(s), C.byref(a), C.byref(b)), "scale_images")
+        return a.value, b.value
+
+    def solve_prepared(self, d_U: int, d_Uinit: int = 0, warps: int = 0, max_iterations: int = 0,
+                       stream: int = 0) -> None:
+        _check(_lib.of_tvl1_solve_prepared(self._h, d_Uinit or None, int(warps), int(max_iterations),
+                                           d_U, stream or None), "solve_prepared")
+
+    def scale_flow(self, s: int, d_Us: int, stream: int = 0) -> None:
+        _check(_lib.of_tvl1_scale_flow(self._h, int(s), d_Us, stream or None), "scale_flow")
+
+    def energy(self, s: int, d_Us: int, stream: int = 0) -> tuple[float, float, float]:
+        """(E, data sum, TV sum) of the device flow d_Us at scale s."""
+        e = (C.c_double * 3)()
+        _check(_lib.of_tvl1_energy(self._h, int(s), d_Us, e, stream or None), "energy")
+        return e[0], e[1], e[2]
 
 
 from .hs import HornSchunck  # noqa: E402

@@ -1,0 +1,62 @@
+"""Batched TV-L1 binding (of_tvl1_batch_*): many device-resident frame pairs
+of mixed sizes solved concurrently."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import TVL1Params, _check, _lib
+
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_lib.of_tvl1_batch_create.restype = C.c_int
+_lib.of_tvl1_batch_create.argtypes = [C.POINTER(TVL1Params), C.c_int, C.POINTER(_vp)]
+_lib.of_tvl1_batch_destroy.restype = None
+_lib.of_tvl1_batch_destroy.argtypes = [_vp]
+_lib.of_tvl1_batch_set_shape.restype = C.c_int
+_lib.of_tvl1_batch_set_shape.argtypes = [_vp, C.c_longlong, C.c_int, C.c_int]
+_lib.of_tvl1_batch_run_u8.restype = C.c_int
+_lib.of_tvl1_batch_run_u8.argtypes = [_vp, C.c_int, _ip, _ip, C.POINTER(_vp), C.POINTER(_vp),
+                                      C.POINTER(_vp), _ip]
+
+
+class TVL1Batch:
+    def __init__(self, streams: int = 4, tau=0.25, lambda_=0.15, theta=0.3, nscales=100,
+                 zfactor=0.5, warps=5, epsilon=0.01, max_iterations=300):
+        p = TVL1Params(tau, lambda_, theta, nscales, zfactor, warps, epsilon, max_iterations)
+        h = _vp()
+        _check(_lib.of_tvl1_batch_create(C.byref(p), int(streams), C.byref(h)), "batch_create")
+        self._h = h
+
+    def set_shape(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int) -> None:
+        _check(_lib.of_tvl1_batch_set_shape(self._h, int(min_pixels), int(iters_per_launch),
+                                            int(rows_per_wave)), "batch_set_shape")
+
+    def run_device_u8(self, widths, heights, d_I0, d_I1, d_U):
+        """d_I0/d_I1/d_U: sequences of device pointers (ints).  Returns the
+        total iteration count of each pair."""
+        n = len(widths)
+        w = np.ascontiguousarray(widths, np.int32)
+        h = np.ascontiguousarray(heights, np.int32)
+        a0 = (_vp * n)(*d_I0)
+        a1 = (_vp * n)(*d_I1)
+        au = (_vp * n)(*d_U)
+        it = np.zeros(n, np.int32)
+        _check(_lib.of_tvl1_batch_run_u8(self._h, n, w.ctypes.data_as(_ip), h.ctypes.data_as(_ip),
+                                         a0, a1, au, it.ctypes.data_as(_ip)), "batch_run_u8")
+        return it
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.of_tvl1_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

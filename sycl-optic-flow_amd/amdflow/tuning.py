@@ -41,8 +41,11 @@ def store(width, height, nscales, warps, shapes, ms_table=None, arch=ARCH) -> No
 
 
 def apply(tv, shapes) -> None:
-    """Installs one launch rule per scale (keyed by the scale's pixel count)."""
+    """Installs one launch rule per scale (keyed by the scale's pixel count);
+    each entry is (K, R) or (K, R, persistent)."""
     tv.clear_launch_rules()
-    for s, (k, r) in enumerate(shapes):
+    for s, shape in enumerate(shapes):
+        k, r = shape[0], shape[1]
+        p = shape[2] if len(shape) > 2 else 0
         nx, ny = tv.scale_size(s)
-        tv.set_launch_rule(nx * ny, int(k), int(r))
+        tv.set_launch_rule(nx * ny, int(k), int(r), bool(p))

@@ -1,0 +1,198 @@
+"""Training and evaluation of the learned TV-L1 warm start.
+
+Data: seeded synthetic 1920x1080 pairs generated on the GPU (of_synth_pair,
+sinusoid + bump motion up to 20 px, noise sigma 1), each solved by the plain
+TV-L1 solver (5 scales, 5 warpings, at most 300 iterations, zero initial
+flow).  The network input is the coarsest scale's (I0, I1, Ix, Iy, It), the
+label is the flow the plain solver ends that scale with.  Training: Adam,
+learning rate 1e-3, mean-squared flow error, on the GPU through the
+library's own forward / backward kernels.
+
+Evaluation: the warm-started solve (at most `warps` warpings and
+`max_iterations` iterations per warping, energy-checked fallback to the
+plain solve) and the same limited solve from zero flow are both compared
+with the converged plain solve by mean end-point error (EPE).
+
+    PYTHONPATH=sycl-optic-flow_amd python -m amdflow.warmstart --pairs 20000 \
+        --epochs 8 --out sycl-optic-flow_amd/weights/warmstart_c32_h5.npy
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import TV_L1, tuning
+from .cnn import WarmStartNet
+from .synth import DevicePairs
+
+WIDTH, HEIGHT, CAP = 1920, 1080, 20.0
+TRAIN_SEED0 = 1_000_000
+EVAL_SEED0 = 9_000_000
+
+
+def _solver(width: int, height: int, **kw) -> TV_L1:
+    tv = TV_L1(width, height, nscales=5, **kw)
+    entry = tuning.lookup(width, height, tv.nscales, int(tv.params.warps))
+    if entry is not None:
+        tuning.apply(tv, [tuple(s) for s in entry["shapes"]])
+    return tv
+
+
+def make_dataset(npairs: int, seed0: int = TRAIN_SEED0, width: int = WIDTH, height: int = HEIGHT,
+                 cap: float = CAP, log=print, every: int = 1000):
+    """Returns (X [n, h*w*5], T [n, 2*h*w], (h, w)) as CUDA tensors."""
+    import torch
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    gen = DevicePairs(width, height, cap, bumps=True)
+    tv = _solver(width, height)
+    ns = tv.nscales
+    wc, hc = tv.scale_size(ns - 1)
+    X = torch.empty((npairs, hc * wc * 5), dtype=torch.float32, device="cuda")
+    T = torch.empty((npairs, 2 * hc * wc), dtype=torch.float32, device="cuda")
+    dU = torch.empty(2 * width * height, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(npairs):
+        d0, d1, _ = gen.generate(seed0 + i, sp)
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
+        a, b = tv.scale_images(ns - 1)
+        WarmStartNet.features(a, b, hc, wc, X[i].data_ptr(), sp)
+        tv.scale_flow(ns - 1, T[i].data_ptr(), sp)
+        if log and (i + 1) % every == 0:
+            log(f"dataset: {i + 1}/{npairs} pairs, {time.perf_counter() - t0:.1f} s")
+    stream.synchronize()
+    tv.close()
+    return X, T, (hc, wc)
+
+
+def train(net: WarmStartNet, X, T, hw, epochs: int, batch: int = 32, lr: float = 1e-3,
+          seed: int = 0, log=print):
+    import torch
+    hc, wc = hw
+    n = X.shape[0]
+    g = torch.Generator().manual_seed(seed)
+    history = []
+    t0 = time.perf_counter()
+    steps = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).cuda()
+        losses = []
+        for k in range(0, n - batch + 1, batch):
+            idx = perm[k:k + batch]
+            xb = X.index_select(0, idx).contiguous()
+            tb = T.index_select(0, idx).contiguous()
+            losses.append(net.train_step(xb.data_ptr(), tb.data_ptr(), batch, hc, wc, lr))
+            steps += 1
+        history.append(float(np.mean(losses)))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: mean loss {history[-1]:.5f}, "
+                f"{time.perf_counter() - t0:.1f} s, {steps} steps")
+    return history
+
+
+def epe(a, b, n: int) -> float:
+    """Mean end-point error between two planar flows of n pixels (tensors)."""
+    d = (a - b).view(2, n)
+    return float(d.square().sum(0).sqrt().mean())
+
+
+def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0, warps: int = 2,
+             max_iterations: int = 100, width: int = WIDTH, height: int = HEIGHT, cap: float = CAP,
+             reps: int = 2):
+    """EPE against the converged plain solve and pairs/s of the plain,
+    warm-started and zero-initialised limited solves over `npairs` pairs."""
+    import torch
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    n = width * height
+    gen = DevicePairs(width, height, cap, bumps=True)
+    pairs = []
+    for i in range(npairs):
+        a, b, f = gen.generate(seed0 + i, sp)
+        stream.synchronize()
+        pairs.append((a.clone(), b.clone(), f.clone()))
+    torch.cuda.synchronize()
+    tv = _solver(width, height)
+    outs = {k: [torch.empty(2 * n, dtype=torch.float32, device="cuda") for _ in range(npairs)]
+            for k in ("plain", "warm", "zero")}
+    fell, e_pred, e_zero, its = [], [], [], {"plain": 0, "warm": 0, "zero": 0}
+
+    def run(kind, i):
+        d0, d1, _ = pairs[i]
+        dU = outs[kind][i].data_ptr()
+        if kind == "plain":
+            tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU, sp)
+        elif kind == "warm":
+            fb, ep, ez = net.warm_run(tv, d0.data_ptr(), d1.data_ptr(), dU, warps, max_iterations, sp)
+            return fb, ep, ez
+        else:
+            tv.prepare_device_u8(d0.data_ptr(), d1.data_ptr(), sp)
+            tv.solve_prepared(dU, 0, warps, max_iterations, sp)
+        return None
+
+    timing = {}
+    for kind in ("plain", "warm", "zero"):
+        for i in range(npairs):                  # warm-up and results
+            r = run(kind, i)
+            its[kind] += int(tv.iterations().sum())
+            if kind == "warm":
+                fell.append(r[0]); e_pred.append(r[1]); e_zero.append(r[2])
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for i in range(npairs):
+                run(kind, i)
+        stream.synchronize()
+        timing[kind] = reps * npairs / (time.perf_counter() - t0)
+    res = {
+        "pairs": npairs, "warps": warps, "max_iterations": max_iterations,
+        "pairs_per_s": {k: round(v, 2) for k, v in timing.items()},
+        "speedup_warm_vs_plain": round(timing["warm"] / timing["plain"], 3),
+        "mean_epe_vs_plain": {
+            "warm": float(np.mean([epe(outs["warm"][i], outs["plain"][i], n) for i in range(npairs)])),
+            "zero_init_same_limits": float(np.mean([epe(outs["zero"][i], outs["plain"][i], n)
+                                                    for i in range(npairs)])),
+        },
+        "mean_epe_vs_ground_truth": {
+            k: float(np.mean([epe(outs[k][i], pairs[i][2], n) for i in range(npairs)]))
+            for k in ("plain", "warm", "zero")},
+        "iterations_per_pair": {k: its[k] / npairs for k in its},
+        "fallbacks": int(sum(fell)),
+        "mean_energy_ratio_pred_vs_zero": float(np.mean(np.asarray(e_pred) / np.asarray(e_zero))),
+    }
+    tv.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--eval-pairs", type=int, default=16)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    import torch
+    torch.cuda.set_device(0)
+    log = lambda m: print(m, flush=True)   # noqa: E731
+    X, T, hw = make_dataset(args.pairs, log=log)
+    net = WarmStartNet(seed=args.seed)
+    log(f"network: {net.num_params} parameters; coarsest scale {hw[1]}x{hw[0]}")
+    hist = train(net, X, T, hw, args.epochs, args.batch, args.lr, args.seed, log=log)
+    if args.out:
+        net.save(args.out)
+        log(f"saved {args.out}")
+    del X, T
+    res = evaluate(net, args.eval_pairs)
+    res["train_loss"] = hist
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

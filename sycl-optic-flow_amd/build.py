@@ -24,10 +24,14 @@ LIB = os.path.join(LIBDIR, "libamdflow.so")
 SOURCES = [
     "common.cpp",
     "tvl1_kernels.hip",
+    "tvl1_iter.hip",
     "tvl1.cpp",
     "debug.cpp",
+    "batch.cpp",
     "hs.hip",
     "lk.hip",
+    "cnn.hip",
+    "synth.hip",
 ]
 
 ARCH = os.environ.get("AMDFLOW_ARCH", "gfx950")

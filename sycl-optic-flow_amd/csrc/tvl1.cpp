@@ -28,18 +28,21 @@ struct LaunchRule {
     long long min_pixels;
     int K;
     int R;
+    int persistent;   // 1: whole warping in one cooperative launch
 };
 
 // Built-in launch shapes per scale size (pixels): large scales are bound by
-// memory traffic and use short temporal blocking; small scales are bound by
-// launch latency and fuse more iterations per launch.
+// memory traffic and use short temporal blocking with host-paced launches;
+// small scales are bound by launch latency and run each warping in one
+// persistent launch.
 const LaunchRule kDefaultRules[] = {
-    {200000, 2, 8},
-    {20000, 4, 4},
-    {0, 4, 4},
+    {200000, 2, 8, 0},
+    {20000, 4, 4, 1},
+    {0, 4, 4, 1},
 };
 
 constexpr int kInFlight = 3;   // iteration launches kept queued ahead of the device
+constexpr int kEnergyBlocks = 1024;
 
 }  // namespace
 
@@ -70,6 +73,10 @@ struct of_tvl1 {
     float* hflow = nullptr;     // pinned host copy returned by of_tvl1_flow
     TvCtl* ctl = nullptr;
     double* partials = nullptr;
+    double* energy = nullptr;   // 2*kEnergyBlocks partials, then (E, data, TV)
+    bool prepared = false;      // pyramids hold the last prepared pair
+    bool solved = false;        // per-scale flows hold the last solve
+    int run_warps = 0, run_max = 0;
     int* d_iters = nullptr;
     unsigned long long* hstatus = nullptr;   // host-mapped
     unsigned epoch = 0;
@@ -85,11 +92,11 @@ struct of_tvl1 {
     std::vector<hipEvent_t> ev_pool;
     struct Timed {
         int ev;          // index of the start event; end event is ev+1
-        int K, R;
+        int K, R, persistent;
     };
     std::vector<Timed> timed;
     std::vector<of_tvl1_kernel_profile> prof;
-    int scale_K[kMaxScales]{}, scale_R[kMaxScales]{};
+    int scale_K[kMaxScales]{}, scale_R[kMaxScales]{}, scale_P[kMaxScales]{};
 
     // per-scale event timing (autotuner)
     bool scale_timing = false;
@@ -99,7 +106,7 @@ struct of_tvl1 {
 
 namespace {
 
-int pick_rule(const of_tvl1* h, long long npx, int* K, int* R)
+int pick_rule(const of_tvl1* h, long long npx, int* K, int* R, int* persistent)
 {
     const LaunchRule* best = nullptr;
     for (const LaunchRule& r : h->rules)
@@ -110,6 +117,7 @@ int pick_rule(const of_tvl1* h, long long npx, int* K, int* R)
     if (!best) best = &kDefaultRules[2];
     *K = best->K;
     *R = best->R;
+    *persistent = best->persistent;
     return OF_OK;
 }
 
@@ -133,6 +141,7 @@ void release(of_tvl1* h)
     dev_free(h->dflow);
     dev_free(h->ctl);
     dev_free(h->partials);
+    dev_free(h->energy);
     dev_free(h->d_iters);
     if (h->hflow) (void)hipHostFree(h->hflow);
     if (h->hstatus) (void)hipHostFree(h->hstatus);
@@ -143,7 +152,7 @@ void release(of_tvl1* h)
 }
 
 // Launch bracketed by two events from the handle's pool.
-int timed_launch(of_tvl1* h, int K, int R, const IterArgs& a, hipStream_t st)
+int timed_launch(of_tvl1* h, int K, int R, int persistent, const IterArgs& a, hipStream_t st)
 {
     const int ev = 2 * (int)h->timed.size();
     while ((int)h->ev_pool.size() < ev + 2) {
@@ -152,38 +161,40 @@ int timed_launch(of_tvl1* h, int K, int R, const IterArgs& a, hipStream_t st)
         h->ev_pool.push_back(e);
     }
     OF_HIP(hipEventRecord(h->ev_pool[ev], st));
-    OF_TRY(launch_iter(K, R, a, st));
+    if (persistent) OF_TRY(launch_persist(K, R, a, 0, st));
+    else OF_TRY(launch_iter(K, R, a, st));
     OF_HIP(hipEventRecord(h->ev_pool[ev + 1], st));
-    h->timed.push_back({ev, K, R});
+    h->timed.push_back({ev, K, R, persistent});
     return OF_OK;
 }
 
-of_tvl1_kernel_profile& prof_entry(of_tvl1* h, int K, int R)
+of_tvl1_kernel_profile& prof_entry(of_tvl1* h, int K, int R, int persistent)
 {
     for (of_tvl1_kernel_profile& p : h->prof)
-        if (p.iters_per_launch == K && p.rows_per_wave == R) return p;
+        if (p.iters_per_launch == K && p.rows_per_wave == R && p.persistent == persistent) return p;
     of_tvl1_kernel_profile p{};
     p.iters_per_launch = K;
     p.rows_per_wave = R;
+    p.persistent = persistent;
     h->prof.push_back(p);
     return h->prof.back();
 }
 
 // After a synchronised solve: fold the timed launches and the completed
-// iterations into the per-shape accumulators.
+// iterations into the per-kernel accumulators.
 int collect_profile(of_tvl1* h)
 {
     for (const of_tvl1::Timed& t : h->timed) {
         float ms = 0.0f;
         OF_HIP(hipEventElapsedTime(&ms, h->ev_pool[t.ev], h->ev_pool[t.ev + 1]));
-        of_tvl1_kernel_profile& p = prof_entry(h, t.K, t.R);
+        of_tvl1_kernel_profile& p = prof_entry(h, t.K, t.R, t.persistent);
         p.launches++;
         p.milliseconds += ms;
     }
     h->timed.clear();
     for (int s = 0; s < h->ns; s++) {
         if (!h->scale_K[s]) continue;
-        of_tvl1_kernel_profile& p = prof_entry(h, h->scale_K[s], h->scale_R[s]);
+        of_tvl1_kernel_profile& p = prof_entry(h, h->scale_K[s], h->scale_R[s], h->scale_P[s]);
         const long long N = (long long)h->nx[s] * h->ny[s];
         for (int w = 0; w < h->prm.warps; w++) {
             const long long pi = (long long)h->h_iters[(size_t)s * h->prm.warps + w] * N;
@@ -199,11 +210,12 @@ int collect_profile(of_tvl1* h)
 int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
 {
     const of_tvl1_params& P = h->prm;
-    if (P.max_iterations <= 0) return OF_OK;
-    int K, R;
-    pick_rule(h, (long long)h->nx[s] * h->ny[s], &K, &R);
+    if (h->run_max <= 0) return OF_OK;
+    int K, R, persistent;
+    pick_rule(h, (long long)h->nx[s] * h->ny[s], &K, &R, &persistent);
     h->scale_K[s] = K;
     h->scale_R[s] = R;
+    h->scale_P[s] = persistent;
     const size_t N = (size_t)h->nx[s] * h->ny[s];
     IterArgs a{};
     a.wx = h->wx;
@@ -221,7 +233,7 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     a.theta = P.theta;
     a.taut = P.tau / P.theta;
     a.eps2 = (double)(P.epsilon * P.epsilon);
-    a.max_iter = P.max_iterations;
+    a.max_iter = h->run_max;
     a.level = s;
     a.ctl = h->ctl;
     a.iters_slot = h->d_iters + (size_t)s * P.warps + w;
@@ -229,7 +241,14 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     a.host_status = h->hstatus;
     a.epoch = ++h->epoch;
 
-    const long long max_launches = (P.max_iterations + K - 1) / K + 1;
+    if (persistent) {
+        // the whole warping in one launch: no host involvement
+        if (h->profiling) return timed_launch(h, K, R, 1, a, st);
+        h->launches_issued++;
+        return launch_persist(K, R, a, 0, st);
+    }
+
+    const long long max_launches = (h->run_max + K - 1) / K + 1;
     long long issued = 0, completed = 0;
     bool stopped = false;
     using clk = std::chrono::steady_clock;
@@ -246,7 +265,7 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
         const auto tl = clk::now();
         bool launched = false;
         while (issued < max_launches && issued - completed < inflight) {
-            if (h->profiling) OF_TRY(timed_launch(h, K, R, a, st));
+            if (h->profiling) OF_TRY(timed_launch(h, K, R, 0, a, st));
             else OF_TRY(launch_iter(K, R, a, st));
             issued++;
             h->launches_issued++;
@@ -290,9 +309,10 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     return OF_OK;
 }
 
-// Enqueues one full multiscale solve from device frames and waits for it.
-// `is_u8` selects the element type of src0/src1.
-int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst, hipStream_t st)
+// Enqueues the first phase of a solve: frames to float, joint
+// normalisation, pre-smoothing and both pyramids.  `is_u8` selects the
+// element type of src0/src1.
+int prepare(of_tvl1* h, const void* src0, const void* src1, bool is_u8, hipStream_t st)
 {
     const of_tvl1_params& P = h->prm;
     const size_t N0 = (size_t)h->W * h->H;
@@ -319,10 +339,28 @@ int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst
                                           h->nx[s], h->ny[s], P.zfactor, st));
         }
     }
-    // zero flow at the coarsest scale, tvl1.cpp:199-200
+    h->prepared = true;
+    h->solved = false;
+    return OF_OK;
+}
+
+// Second phase: the coarse-to-fine solve from the prepared pyramids, with
+// `init` (planar [u1|u2] at the coarsest scale, device) as the initial flow,
+// or zero when null (tvl1.cpp:199-200); at most `warps` warpings per scale
+// and `max_iter` iterations per warping.  Waits for the result.
+int solve_levels(of_tvl1* h, const float* init, int warps, int max_iter, float* dst, hipStream_t st)
+{
+    const of_tvl1_params& P = h->prm;
+    const size_t N0 = (size_t)h->W * h->H;
+    h->run_warps = warps;
+    h->run_max = max_iter;
+    OF_HIP(hipMemsetAsync(h->d_iters, 0, sizeof(int) * h->h_iters.size(), st));
     {
         const int s = h->ns - 1;
-        OF_TRY(launch_fill(h->ubase + 4 * h->off[s], 2 * (size_t)h->nx[s] * h->ny[s], 0.0f, st));
+        const size_t n2 = 2 * (size_t)h->nx[s] * h->ny[s];
+        float* uc = h->ubase + 4 * h->off[s];   // buffer 0: launch_ctl_init selected it
+        if (init) OF_HIP(hipMemcpyAsync(uc, init, sizeof(float) * n2, hipMemcpyDeviceToDevice, st));
+        else OF_TRY(launch_fill(uc, n2, 0.0f, st));
     }
     const float invZ = 1 / P.zfactor;
     for (int s = h->ns - 1; s >= 0; s--) {
@@ -334,7 +372,7 @@ int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst
         if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s], st));
         OF_TRY(launch_centered_gradient(I1s, h->I1x, h->I1y, nx, ny, st));
         OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:232-235
-        for (int w = 0; w < P.warps; w++) {
+        for (int w = 0; w < warps; w++) {
             OF_TRY(launch_warp_setup(I0s, I1s, h->I1x, h->I1y, ub, ub + 2 * N, h->wx, h->wy,
                                      h->rhoc, nx, ny, h->ctl, s, st));
             OF_TRY(run_warping(h, s, w, st));
@@ -346,10 +384,19 @@ int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst
         }
         if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s + 1], st));
     }
-    OF_TRY(launch_copy_flow(h->ubase, h->ubase + 2 * N0, dst, 2 * N0, h->ctl, st));
+    OF_TRY(launch_copy_flow(h->ubase, h->ubase + 2 * N0, dst, 2 * N0, h->ctl, 0, st));
     OF_HIP(hipMemcpyAsync(h->h_iters.data(), h->d_iters, sizeof(int) * h->h_iters.size(),
                           hipMemcpyDeviceToHost, st));
     OF_HIP(hipStreamSynchronize(st));
+    {
+        // a persistent warping whose grid barrier ran past its bound flags it
+        int derr = 0;
+        OF_HIP(hipMemcpy(&derr, &h->ctl->error, sizeof(int), hipMemcpyDeviceToHost));
+        if (derr) {
+            record_hip_error(hipErrorLaunchTimeOut, "k_tvl1_persist grid barrier", __FILE__, __LINE__);
+            return OF_ERR_HIP;
+        }
+    }
     if (h->profiling) OF_TRY(collect_profile(h));
     if (h->scale_timing)
         for (int s = 0; s < h->ns; s++) {
@@ -357,7 +404,15 @@ int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst
             OF_HIP(hipEventElapsedTime(&ms, h->scale_ev[2 * s], h->scale_ev[2 * s + 1]));
             h->scale_ms[s] += ms;
         }
+    h->solved = true;
     return OF_OK;
+}
+
+// One full multiscale solve with the handle's parameters and zero initial flow.
+int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst, hipStream_t st)
+{
+    OF_TRY(prepare(h, src0, src1, is_u8, st));
+    return solve_levels(h, nullptr, h->prm.warps, h->prm.max_iterations, dst, st);
 }
 
 // Launch-shape autotuner: every supported (iterations per launch, rows per
@@ -379,36 +434,45 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
         for (hipEvent_t& e : h->scale_ev) OF_HIP(hipEventCreate(&e));
     }
     std::vector<double> best(h->ns, 1e30);
-    std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8);
+    std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8), bestP(h->ns, 0);
     const std::vector<LaunchRule> saved = h->rules;
-    for (int c = 0; c < nshape; c++) {
-        h->rules.assign(1, LaunchRule{0, kShapes[c][0], kShapes[c][1]});
-        OF_TRY(solve(h, d0, d1, true, h->dflow, h->stream));   // warm-up of this shape
-        for (int s = 0; s < h->ns; s++) h->scale_ms[s] = 0.0;
-        h->scale_timing = true;
-        int st = OF_OK;
-        for (int r = 0; r < reps && st == OF_OK; r++) st = solve(h, d0, d1, true, h->dflow, h->stream);
-        h->scale_timing = false;
-        if (st != OF_OK) {
-            h->rules = saved;
-            return st;
-        }
-        for (int s = 0; s < h->ns; s++) {
-            const double ms = h->scale_ms[s] / reps;
-            if (table) table[(size_t)c * h->ns + s] = ms;
-            if (ms < best[s]) {
-                best[s] = ms;
-                bestK[s] = kShapes[c][0];
-                bestR[s] = kShapes[c][1];
+    for (int mode = 0; mode < 2; mode++) {
+        for (int c = 0; c < nshape; c++) {
+            const int row = mode * nshape + c;
+            h->rules.assign(1, LaunchRule{0, kShapes[c][0], kShapes[c][1], mode});
+            int st = solve(h, d0, d1, true, h->dflow, h->stream);   // warm-up of this variant
+            if (st == OF_ERR_UNSUPPORTED) {   // no cooperative launch on this device
+                if (table)
+                    for (int s = 0; s < h->ns; s++) table[(size_t)row * h->ns + s] = -1.0;
+                continue;
+            }
+            for (int s = 0; s < h->ns; s++) h->scale_ms[s] = 0.0;
+            h->scale_timing = true;
+            for (int r = 0; r < reps && st == OF_OK; r++) st = solve(h, d0, d1, true, h->dflow, h->stream);
+            h->scale_timing = false;
+            if (st != OF_OK) {
+                h->rules = saved;
+                return st;
+            }
+            for (int s = 0; s < h->ns; s++) {
+                const double ms = h->scale_ms[s] / reps;
+                if (table) table[(size_t)row * h->ns + s] = ms;
+                if (ms < best[s]) {
+                    best[s] = ms;
+                    bestK[s] = kShapes[c][0];
+                    bestR[s] = kShapes[c][1];
+                    bestP[s] = mode;
+                }
             }
         }
     }
     h->rules.clear();
     for (int s = 0; s < h->ns; s++) {
-        h->rules.push_back(LaunchRule{(long long)h->nx[s] * h->ny[s], bestK[s], bestR[s]});
+        h->rules.push_back(LaunchRule{(long long)h->nx[s] * h->ny[s], bestK[s], bestR[s], bestP[s]});
         if (chosen) {
-            chosen[2 * s] = bestK[s];
-            chosen[2 * s + 1] = bestR[s];
+            chosen[3 * s] = bestK[s];
+            chosen[3 * s + 1] = bestR[s];
+            chosen[3 * s + 2] = bestP[s];
         }
     }
     return OF_OK;
@@ -518,7 +582,9 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
     if (!st) st = dev_alloc(&h->prev, N0);
     if (!st) st = dev_alloc(&h->dflow, 2 * N0);
     if (!st) st = dev_alloc(&h->ctl, 1);
-    if (!st) st = dev_alloc(&h->partials, maxpart);
+    // x2: the persistent kernel double-buffers its partials by chunk parity
+    if (!st) st = dev_alloc(&h->partials, 2 * maxpart);
+    if (!st) st = dev_alloc(&h->energy, 2 * kEnergyBlocks + 3);
     if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
     if (!st && hipHostMalloc((void**)&h->hflow, sizeof(float) * 2 * N0, hipHostMallocDefault) != hipSuccess) {
         h->hflow = nullptr;
@@ -566,6 +632,64 @@ int of_tvl1_run_device_f32(of_tvl1* h, const float* dI0, const float* dI1, float
     if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
     hipStream_t st = stream ? (hipStream_t)stream : h->stream;
     return solve(h, dI0, dI1, false, dU, st);
+}
+
+int of_tvl1_prepare_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, void* stream)
+{
+    if (!h || !dI0 || !dI1) return OF_ERR_ARG;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    return prepare(h, dI0, dI1, true, st);
+}
+
+int of_tvl1_scale_images(const of_tvl1* h, int s, const float** dI0s, const float** dI1s)
+{
+    if (!h || s < 0 || s >= h->ns || !dI0s || !dI1s) return OF_ERR_ARG;
+    if (!h->prepared) return OF_ERR_STATE;
+    *dI0s = h->I0p + h->off[s];
+    *dI1s = h->I1p + h->off[s];
+    return OF_OK;
+}
+
+int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int warps, int max_iterations, float* dU,
+                           void* stream)
+{
+    if (!h || !dU || warps < 0 || warps > h->prm.warps || max_iterations < 0) return OF_ERR_ARG;
+    if (!h->prepared) return OF_ERR_STATE;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    if (warps == 0) warps = h->prm.warps;
+    if (max_iterations == 0) max_iterations = h->prm.max_iterations;
+    return solve_levels(h, dUinit, warps, max_iterations, dU, st);
+}
+
+int of_tvl1_scale_flow(of_tvl1* h, int s, float* dUs, void* stream)
+{
+    if (!h || s < 0 || s >= h->ns || !dUs) return OF_ERR_ARG;
+    if (!h->solved) return OF_ERR_STATE;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    const size_t N = (size_t)h->nx[s] * h->ny[s];
+    float* ub = h->ubase + 4 * h->off[s];
+    OF_TRY(launch_copy_flow(ub, ub + 2 * N, dUs, 2 * N, h->ctl, s, st));
+    OF_HIP(hipStreamSynchronize(st));
+    return OF_OK;
+}
+
+int of_tvl1_energy(of_tvl1* h, int s, const float* dUs, double* energy, void* stream)
+{
+    if (!h || s < 0 || s >= h->ns || !dUs) return OF_ERR_ARG;
+    if (!h->prepared) return OF_ERR_STATE;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    const size_t N = (size_t)h->nx[s] * h->ny[s];
+    OF_TRY(launch_energy(h->I0p + h->off[s], h->I1p + h->off[s], dUs, dUs + N, h->nx[s], h->ny[s],
+                         h->prm.lambda, h->energy, kEnergyBlocks, h->energy + 2 * kEnergyBlocks, st));
+    double e[3];
+    OF_HIP(hipMemcpyAsync(e, h->energy + 2 * kEnergyBlocks, sizeof(e), hipMemcpyDeviceToHost, st));
+    OF_HIP(hipStreamSynchronize(st));
+    if (energy) {
+        energy[0] = e[0];
+        energy[1] = e[1];
+        energy[2] = e[2];
+    }
+    return OF_OK;
 }
 
 int of_tvl1_run_u8(of_tvl1* h, const uint8_t* I0, const uint8_t* I1)
@@ -635,7 +759,7 @@ int of_tvl1_get_stats(const of_tvl1* h, of_tvl1_stats* st)
         for (int w = 0; w < h->prm.warps; w++) pi += (long long)h->h_iters[(size_t)s * h->prm.warps + w] * N;
     }
     st->pixel_iterations = pi;
-    st->pixel_warps = (long long)h->prm.warps * sumN;
+    st->pixel_warps = (long long)(h->solved ? h->run_warps : h->prm.warps) * sumN;
     st->pixels_all_scales = sumN;
     st->launches = h->launches_issued;
     st->host_launch_seconds = h->host_launch_s;
@@ -678,7 +802,8 @@ int of_tvl1_profile_get(const of_tvl1* h, of_tvl1_kernel_profile* out, int cap, 
     return OF_OK;
 }
 
-int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_launch, int rows_per_wave)
+int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_launch, int rows_per_wave,
+                            int persistent)
 {
     if (!h) return OF_ERR_ARG;
     if (min_pixels < 0) {
@@ -686,13 +811,15 @@ int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_laun
         return OF_OK;
     }
     if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
+    const int p = persistent ? 1 : 0;
     for (LaunchRule& r : h->rules)
         if (r.min_pixels == min_pixels) {
             r.K = iters_per_launch;
             r.R = rows_per_wave;
+            r.persistent = p;
             return OF_OK;
         }
-    h->rules.push_back({min_pixels, iters_per_launch, rows_per_wave});
+    h->rules.push_back({min_pixels, iters_per_launch, rows_per_wave, p});
     return OF_OK;
 }
 

@@ -1,0 +1,565 @@
+// tvl1_iter.hip -- the TV-L1 primal-dual iterations (tvl1.cpp:256-334).
+//
+// Two drivers share one tile core:
+//   k_tvl1_iter<K,R>     one launch = K iterations; the last workgroup to
+//                        finish evaluates the stopping rule; the host keeps a
+//                        few launches queued (large scales);
+//   k_tvl1_persist<K,R>  one cooperative launch = a whole warping; chunks of
+//                        K iterations separated by a grid barrier, every
+//                        workgroup evaluating the stopping rule on the same
+//                        data (small, launch-latency-bound scales).
+// Arithmetic follows the reference operation by operation; the library is
+// built with -ffp-contract=off.
+#include "device_math.hpp"
+#include "tvl1_kernels.hpp"
+
+namespace amdflow {
+
+// Divergence at one pixel from the current value and the left/up neighbours
+// with the border rules of mask.cpp:44-90.  Written with selects; every
+// candidate is the exact expression of its reference case (the first/last
+// column cases associate left to right as written at :81-82).
+__device__ __forceinline__ float div_rule(float v1, float v1l, float v2, float v2u, bool fr,
+                                          bool lr, bool fc, bool lc)
+{
+    const float a = fc ? v1 : (lc ? -v1l : (v1 - v1l));
+    const float t_first_row = a + v2;               // :70, corners :86-87
+    const float t_last_row = a - v2u;               // :71, corners :88-89
+    const float t_edge_col = (a + v2) - v2u;        // :81-82
+    const float t_body = a + (v2 - v2u);            // :59
+    return fr ? t_first_row : (lr ? t_last_row : ((fc || lc) ? t_edge_col : t_body));
+}
+
+// Thresholding operator TH, tvl1.cpp:263-287, branch-free: the three
+// reference cases multiply the warped gradient by l_t, -l_t or -rho/grad,
+// and the `grad < 1e-10` case stores +0 exactly as the reference does.
+__device__ __forceinline__ void th_step(float rc, float gx, float gy, float u1, float u2,
+                                        float l_t, float& v1, float& v2)
+{
+    const float grad = (gx * gx) + (gy * gy);
+    const float rho = rc + ((gx * u1) + (gy * u2));
+    const float fi = -rho / grad;
+    const bool c1 = rho >= -l_t * grad;
+    const bool c2 = rho > l_t * grad;
+    const bool c3 = grad < 1E-10f;
+    const bool zero = c1 && !c2 && c3;
+    const float k = c1 ? (c2 ? -l_t : fi) : l_t;
+    const float d1 = zero ? 0.0f : k * gx;
+    const float d2 = zero ? 0.0f : k * gy;
+    v1 = u1 + d1;
+    v2 = u2 + d2;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Loads and stores of the iteration state.  COH: device-scope atomic
+// accesses, coherent across XCDs without cache maintenance (used when other
+// workgroups of the same launch read the values).
+template <bool COH>
+__device__ __forceinline__ float ld(const float* p)
+{
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <bool COH>
+__device__ __forceinline__ void st(float* p, float v)
+{
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+struct TileShared {
+    float a[4][64], b[4][64], c[4][64], d[4][64];
+};
+
+// kk (<= K) iterations on tile (bx, by): reads buffer `cur`, writes the
+// central tile into buffer cur^1, adds each iteration's per-lane error sum
+// over the central tile to acc[t].
+//
+// One workgroup = 4 waves; the extended tile is 64 columns (one per lane) by
+// 4*R rows (R consecutive rows per wave, in registers).  Horizontal
+// neighbours come from DPP lane shifts, vertical ones from the lane's own
+// registers or, across waves, from one LDS row.  Values within K pixels of
+// the tile edge go stale one pixel per iteration; the central (64-2K) x
+// (4R-2K) pixels are exact.
+template <int K, int R, bool COH>
+__device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, int cur, int kk,
+                                             double (&acc)[K], TileShared& sh)
+{
+    constexpr int NW = 4;
+    constexpr int TW = 64 - 2 * K;
+    constexpr int TH = NW * R - 2 * K;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
+    const int nx = a.nx, ny = a.ny;
+    const int x = bx * TW - K + lane;
+    const int ybase = by * TH - K + wave * R;
+    const int oy0 = by * TH;
+    const bool xin = x >= 0 && x < nx;
+    const bool xout = xin && lane >= K && lane < K + TW;
+    const bool fc = x == 0, lc = x == nx - 1;
+    // waves holding no first/last image column take the body formula of the
+    // divergence on every row away from the first/last image row
+    const bool edge_cols = __any(fc || lc);
+
+    const float* uin = a.u[cur];
+    const float* pin = a.p[cur];
+    const size_t plane = (size_t)a.plane;
+
+    float u1[R], u2[R], p11[R], p12[R], p21[R], p22[R], gx[R], gy[R], rc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = ybase + r;
+        if (xin && y >= 0 && y < ny) {
+            const size_t idx = (size_t)y * nx + x;
+            u1[r] = ld<COH>(uin + idx);
+            u2[r] = ld<COH>(uin + plane + idx);
+            p11[r] = ld<COH>(pin + idx);
+            p12[r] = ld<COH>(pin + plane + idx);
+            p21[r] = ld<COH>(pin + 2 * plane + idx);
+            p22[r] = ld<COH>(pin + 3 * plane + idx);
+            gx[r] = a.wx[idx];
+            gy[r] = a.wy[idx];
+            rc[r] = a.rhoc[idx];
+        } else {
+            u1[r] = u2[r] = p11[r] = p12[r] = p21[r] = p22[r] = 0.0f;
+            gx[r] = gy[r] = rc[r] = 0.0f;
+        }
+    }
+
+    const float l_t = a.l_t, theta = a.theta, taut = a.taut;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < kk) {
+            // ---- u update: TH, divergence of p, u = v + theta*div p ----
+            sh.a[wave][lane] = p12[R - 1];
+            sh.b[wave][lane] = p22[R - 1];
+            __syncthreads();
+            const float up12 = wave ? sh.a[wave - 1][lane] : p12[0];
+            const float up22 = wave ? sh.b[wave - 1][lane] : p22[0];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int y = ybase + r;
+                const float p11l = from_left(p11[r]);
+                const float p21l = from_left(p21[r]);
+                const float p12u = r ? p12[r - 1] : up12;
+                const float p22u = r ? p22[r - 1] : up22;
+                const bool fr = y == 0, lr = y == ny - 1;
+                float div1, div2;
+                if (!edge_cols && !fr && !lr) {   // uniform branch: interior rows of interior waves
+                    div1 = (p11[r] - p11l) + (p12[r] - p12u);
+                    div2 = (p21[r] - p21l) + (p22[r] - p22u);
+                } else {
+                    div1 = div_rule(p11[r], p11l, p12[r], p12u, fr, lr, fc, lc);
+                    div2 = div_rule(p21[r], p21l, p22[r], p22u, fr, lr, fc, lc);
+                }
+                float v1, v2;
+                th_step(rc[r], gx[r], gy[r], u1[r], u2[r], l_t, v1, v2);
+                const float nu1 = v1 + theta * div1;
+                const float nu2 = v2 + theta * div2;
+                const float e1 = nu1 - u1[r], e2 = nu2 - u2[r];
+                const float e = e1 * e1 + e2 * e2;
+                const bool mine = xout && y >= oy0 && y < oy0 + TH && y < ny;
+                acc[t] += mine ? (double)e : 0.0;
+                u1[r] = nu1;
+                u2[r] = nu2;
+            }
+            // ---- p update: forward gradient of u, g, p = (p + taut*grad)/g ----
+            sh.c[wave][lane] = u1[0];
+            sh.d[wave][lane] = u2[0];
+            __syncthreads();
+            const float dn1 = wave < NW - 1 ? sh.c[wave + 1][lane] : u1[R - 1];
+            const float dn2 = wave < NW - 1 ? sh.d[wave + 1][lane] : u2[R - 1];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int y = ybase + r;
+                const float u1r = from_right(u1[r]);
+                const float u2r = from_right(u2[r]);
+                const float u1d = r < R - 1 ? u1[r + 1] : dn1;
+                const float u2d = r < R - 1 ? u2[r + 1] : dn2;
+                const bool xl = x < nx - 1, yl = y < ny - 1;
+                const float fx1 = xl ? u1r - u1[r] : 0.0f;
+                const float fy1 = yl ? u1d - u1[r] : 0.0f;
+                const float fx2 = xl ? u2r - u2[r] : 0.0f;
+                const float fy2 = yl ? u2d - u2[r] : 0.0f;
+                const float g1 = 1.0f + taut * hypot_ref(fx1, fy1);
+                const float g2 = 1.0f + taut * hypot_ref(fx2, fy2);
+                // cblas_saxpy (p += taut*grad) then the division, tvl1.cpp:322-333
+                p11[r] = (p11[r] + taut * fx1) / g1;
+                p12[r] = (p12[r] + taut * fy1) / g1;
+                p21[r] = (p21[r] + taut * fx2) / g2;
+                p22[r] = (p22[r] + taut * fy2) / g2;
+            }
+        }
+    }
+
+    float* uout = a.u[cur ^ 1];
+    float* pout = a.p[cur ^ 1];
+    if (xout) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int y = ybase + r;
+            if (y >= oy0 && y < oy0 + TH && y < ny) {
+                const size_t idx = (size_t)y * nx + x;
+                st<COH>(uout + idx, u1[r]);
+                st<COH>(uout + plane + idx, u2[r]);
+                st<COH>(pout + idx, p11[r]);
+                st<COH>(pout + plane + idx, p12[r]);
+                st<COH>(pout + 2 * plane + idx, p21[r]);
+                st<COH>(pout + 3 * plane + idx, p22[r]);
+            }
+        }
+    }
+}
+
+// Workgroup sum of the per-lane partials, fixed order: wave tree, then
+// waves 0..3.  Result valid in thread t for t < K (thread t holds sum t).
+template <int K>
+__device__ __forceinline__ double block_sum(const double (&acc)[K], double (*sh_err)[K], int t_out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const double v = wave_sum_f64(acc[t]);
+        if (lane == 0) sh_err[wave][t] = v;
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (t_out < K) s = ((sh_err[0][t_out] + sh_err[1][t_out]) + sh_err[2][t_out]) + sh_err[3][t_out];
+    __syncthreads();
+    return s;
+}
+
+// Sum over all workgroups of column t of `part` (nb entries), fixed order:
+// thread j sums entries j, j+256, ...; then wave trees and waves 0..3.
+// Identical in every workgroup that calls it on the same data.
+__device__ __forceinline__ double grid_sum(const double* part, unsigned nb, double* sh4)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double s = 0.0;
+    for (unsigned j = threadIdx.x; j < nb; j += blockDim.x)
+        s += __hip_atomic_load(part + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum_f64(s);
+    if (lane == 0) sh4[wave] = s;
+    __syncthreads();
+    const double tot = ((sh4[0] + sh4[1]) + sh4[2]) + sh4[3];
+    __syncthreads();
+    return tot;
+}
+
+// The stopping rule applied to kk error sums after iteration n0 (1-based
+// iterations n0+1 .. n0+kk): returns the number of iterations to keep when
+// the warping stops inside or at the end of this chunk, 0 to continue.
+// tvl1.cpp:256-258: continue while error > epsilon^2 and n < max.
+__device__ __forceinline__ int stop_after(const double* tot, int kk, int n0, const IterArgs& a)
+{
+    for (int t = 0; t < kk; t++) {
+        const double err = tot[t] / (double)a.plane;
+        if (!(err > a.eps2)) return t + 1;
+    }
+    if (n0 + kk >= a.max_iter) return kk;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// launch-per-chunk driver
+// ---------------------------------------------------------------------------
+
+// Stopping rule, evaluated by the last workgroup to finish: when the first
+// iteration that meets it falls inside the launch before its last
+// iteration, the launch's output is discarded and the next launch re-runs
+// exactly that many iterations from the same input buffer.
+template <int K, int R>
+__global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
+{
+    __shared__ TileShared sh;
+    __shared__ double sh_err[4][K];
+    __shared__ double sh_tot[K];
+    __shared__ double sh4[4];
+    __shared__ int sh_last;
+
+    TvCtl* ctl = a.ctl;
+    if (ctl->stop) return;
+    const int cur = ctl->cur[a.level];
+    const int redo = ctl->redo;
+    const int n0 = ctl->n;
+    int kk = redo ? redo : a.max_iter - n0;
+    if (kk > K) kk = K;
+    if (kk <= 0) return;
+
+    double acc[K];
+#pragma unroll
+    for (int t = 0; t < K; t++) acc[t] = 0.0;
+    tile_iterate<K, R, false>(a, blockIdx.x, blockIdx.y, cur, kk, acc, sh);
+
+    const unsigned nblocks = gridDim.x * gridDim.y;
+    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const double part = block_sum<K>(acc, sh_err, threadIdx.x);
+    // The partials are published with device-scope atomic stores, which are
+    // written through to the point of coherence of all XCDs, and every wave
+    // waits for their completion (s_waitcnt vmcnt(0)) before the counter
+    // increment; the last workgroup reads them with device-scope atomic
+    // loads.  A device-scope release fence would instead write back the
+    // XCD's whole L2 at the end of every workgroup.
+    if (threadIdx.x < (unsigned)kk)
+        __hip_atomic_store(a.partials + (size_t)threadIdx.x * nblocks + bid, part, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (threadIdx.x == 0) sh_last = atomicAdd(&ctl->counter, 1u) == nblocks - 1;
+    __syncthreads();
+    if (!sh_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int t = 0; t < kk; t++) {
+        const double tot = grid_sum(a.partials + (size_t)t * nblocks, nblocks, sh4);
+        if (threadIdx.x == 0) sh_tot[t] = tot;
+    }
+    __syncthreads();
+    if (a.err_log && threadIdx.x < (unsigned)kk) a.err_log[n0 + threadIdx.x] = sh_tot[threadIdx.x];
+    if (threadIdx.x == 0) {
+        int n = n0, stop = 0, nredo = 0, flip = 0;
+        if (redo) {
+            n = n0 + redo;
+            stop = 1;
+            flip = 1;
+        } else {
+            const int keep = stop_after(sh_tot, kk, n0, a);
+            if (keep == kk || keep == 0) {
+                n = n0 + kk;
+                flip = 1;
+                stop = keep != 0;
+            } else {
+                nredo = keep;
+            }
+        }
+        ctl->n = n;
+        ctl->redo = nredo;
+        ctl->stop = stop;
+        if (flip) ctl->cur[a.level] = cur ^ 1;
+        if (stop) *a.iters_slot = n;
+        ctl->counter = 0;
+        ctl->launches = ctl->launches + 1;
+        const unsigned wl = ctl->wlaunches + 1;
+        ctl->wlaunches = wl;
+        const unsigned long long word = ((unsigned long long)a.epoch << 32) |
+                                        ((unsigned long long)stop << 31) |
+                                        (unsigned long long)(wl & 0x7FFFFFFFu);
+        // the host only reads this word itself: no ordering with other data
+        if (a.host_status)
+            __hip_atomic_store(a.host_status, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// persistent driver
+// ---------------------------------------------------------------------------
+
+// Grid barrier over `nb` co-resident workgroups (cooperative launch).  Every
+// wave first waits for its own memory operations to complete, so all state
+// written by the workgroup (device-scope stores) is visible at the point of
+// coherence before the arrival.  The wait is bounded: past ~1 s it flags
+// ctl->error and returns false, and the kernel ends.
+__device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& gen, int* sh_ok)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned g = gen;
+        const unsigned old =
+            __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == nb - 1) {
+            __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __hip_atomic_store(&ctl->bar_gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 21)) {
+                    __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        gen = g + 1;
+        *sh_ok = ok;
+    }
+    __syncthreads();
+    return *sh_ok != 0;
+}
+
+// A whole warping: chunks of K iterations over all tiles (workgroup b takes
+// tiles b, b+nb, ...), a grid barrier, then every workgroup sums the chunk's
+// per-workgroup error partials in the same fixed order and takes the same
+// stopping decision.  A stop inside a chunk re-runs exactly that many
+// iterations from the chunk's input buffer.  The state is read and written
+// with device-scope atomic accesses so that halos written by workgroups on
+// other XCDs are seen without cache maintenance.  Partials are
+// double-buffered by chunk parity.
+template <int K, int R>
+__global__ __launch_bounds__(256) void k_tvl1_persist(IterArgs a, unsigned tiles_x, unsigned ntiles)
+{
+    __shared__ TileShared sh;
+    __shared__ double sh_err[4][K];
+    __shared__ double sh_tot[K];
+    __shared__ double sh4[4];
+    __shared__ int sh_ok;
+
+    TvCtl* ctl = a.ctl;
+    const unsigned nb = gridDim.x;
+    unsigned gen = 0;
+    if (threadIdx.x == 0) gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int cur = ctl->cur[a.level];
+    int n = 0;
+    int redo = 0;
+    int parity = 0;
+    while (true) {
+        int kk = redo ? redo : a.max_iter - n;
+        if (kk > K) kk = K;
+        if (kk <= 0) break;
+        double acc[K];
+#pragma unroll
+        for (int t = 0; t < K; t++) acc[t] = 0.0;
+        for (unsigned tile = blockIdx.x; tile < ntiles; tile += nb)
+            tile_iterate<K, R, true>(a, (int)(tile % tiles_x), (int)(tile / tiles_x), cur, kk, acc, sh);
+        if (redo) {   // the re-run chunk is final: no decision needed
+            n += redo;
+            cur ^= 1;
+            break;
+        }
+        const double part = block_sum<K>(acc, sh_err, threadIdx.x);
+        double* pp = a.partials + (size_t)parity * K * nb;
+        if (threadIdx.x < (unsigned)kk)
+            __hip_atomic_store(pp + (size_t)threadIdx.x * nb + blockIdx.x, part, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (!grid_barrier(ctl, nb, gen, &sh_ok)) return;
+        for (int t = 0; t < kk; t++) {
+            const double tot = grid_sum(pp + (size_t)t * nb, nb, sh4);
+            if (threadIdx.x == 0) sh_tot[t] = tot;
+        }
+        __syncthreads();
+        const int keep = stop_after(sh_tot, kk, n, a);
+        if (a.err_log && blockIdx.x == 0 && threadIdx.x < (unsigned)kk)
+            a.err_log[n + threadIdx.x] = sh_tot[threadIdx.x];
+        parity ^= 1;
+        if (keep == 0) {
+            n += kk;
+            cur ^= 1;
+            continue;
+        }
+        if (keep == kk) {
+            n += kk;
+            cur ^= 1;
+            break;
+        }
+        redo = keep;   // state of buffer `cur` is unchanged: re-run `keep` iterations
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->cur[a.level] = cur;
+        ctl->n = n;
+        ctl->stop = 1;
+        *a.iters_slot = n;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+bool iter_config_ok(int K, int R)
+{
+    if (!(K == 1 || K == 2 || K == 4 || K == 8)) return false;
+    if (!(R == 2 || R == 4 || R == 8)) return false;
+    return 4 * R - 2 * K >= 4;
+}
+
+void iter_grid(int K, int R, int nx, int ny, unsigned* gx, unsigned* gy)
+{
+    *gx = cdiv((size_t)nx, (size_t)(64 - 2 * K));
+    *gy = cdiv((size_t)ny, (size_t)(4 * R - 2 * K));
+}
+
+template <int K, int R>
+static hipError_t launch_kr(const IterArgs& a, hipStream_t s)
+{
+    unsigned gx, gy;
+    iter_grid(K, R, a.nx, a.ny, &gx, &gy);
+    hipLaunchKernelGGL((k_tvl1_iter<K, R>), dim3(gx, gy), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+int launch_iter(int K, int R, const IterArgs& a, hipStream_t s)
+{
+    hipError_t e = hipErrorInvalidValue;
+#define OF_CASE(k, r) \
+    if (K == k && R == r) e = launch_kr<k, r>(a, s);
+    OF_CASE(1, 2) OF_CASE(2, 2)
+    OF_CASE(1, 4) OF_CASE(2, 4) OF_CASE(4, 4)
+    OF_CASE(1, 8) OF_CASE(2, 8) OF_CASE(4, 8) OF_CASE(8, 8)
+#undef OF_CASE
+    if (e != hipSuccess) {
+        record_hip_error(e, "k_tvl1_iter launch", __FILE__, __LINE__);
+        return OF_ERR_HIP;
+    }
+    return OF_OK;
+}
+
+template <int K, int R>
+static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
+{
+    static int resident = -1;   // co-resident workgroups of this kernel on the device
+    if (resident < 0) {
+        int dev = 0, per_cu = 0, cus = 0, coop = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tvl1_persist<K, R>,
+                                                         256, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return OF_ERR_HIP;
+        }
+        resident = coop ? per_cu * cus : 0;
+    }
+    if (resident <= 0) return OF_ERR_UNSUPPORTED;
+    unsigned gx, gy;
+    iter_grid(K, R, a.nx, a.ny, &gx, &gy);
+    const unsigned ntiles = gx * gy;
+    unsigned nb = nblocks > 0 ? (unsigned)nblocks : ntiles;
+    if (nb > ntiles) nb = ntiles;
+    if (nb > (unsigned)resident) nb = (unsigned)resident;
+    IterArgs args = a;
+    unsigned tx = gx, nt = ntiles;
+    void* kargs[] = {(void*)&args, (void*)&tx, (void*)&nt};
+    const hipError_t e = hipLaunchCooperativeKernel((const void*)k_tvl1_persist<K, R>, dim3(nb),
+                                                    dim3(256), kargs, 0, s);
+    if (e != hipSuccess) {
+        record_hip_error(e, "k_tvl1_persist cooperative launch", __FILE__, __LINE__);
+        return OF_ERR_HIP;
+    }
+    return OF_OK;
+}
+
+int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s)
+{
+#define OF_PCASE(k, r) \
+    if (K == k && R == r) return launch_persist_kr<k, r>(a, nblocks, s);
+    OF_PCASE(1, 2) OF_PCASE(2, 2)
+    OF_PCASE(1, 4) OF_PCASE(2, 4) OF_PCASE(4, 4)
+    OF_PCASE(1, 8) OF_PCASE(2, 8) OF_PCASE(4, 8) OF_PCASE(8, 8)
+#undef OF_PCASE
+    return OF_ERR_UNSUPPORTED;
+}
+
+}  // namespace amdflow

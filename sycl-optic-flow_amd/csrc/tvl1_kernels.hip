@@ -26,6 +26,8 @@ __global__ void k_ctl_init(TvCtl* ctl)
         ctl->wlaunches = 0;
         ctl->maxkey[0] = ctl->maxkey[1] = 0ull;
         ctl->minkey[0] = ctl->minkey[1] = ~0ull;
+        ctl->bar_count = 0;
+        ctl->error = 0;
     }
 }
 
@@ -316,299 +318,11 @@ __global__ __launch_bounds__(256) void k_warp_setup(const float* __restrict__ I0
 __global__ __launch_bounds__(256) void k_copy_flow(const float* __restrict__ ub0,
                                                    const float* __restrict__ ub1,
                                                    float* __restrict__ out, unsigned n2,
-                                                   const TvCtl* ctl)
+                                                   const TvCtl* ctl, int level)
 {
-    const float* ub = ctl->cur[0] ? ub1 : ub0;
+    const float* ub = ctl->cur[level] ? ub1 : ub0;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x)
         out[i] = ub[i];
-}
-
-// ---------------------------------------------------------------------------
-// primal-dual iterations (tvl1.cpp:256-334), K iterations per launch
-// ---------------------------------------------------------------------------
-
-// Divergence at one pixel from the current value and the left/up neighbours
-// with the border rules of mask.cpp:44-90 (first/last column cases
-// associate left to right as written there).
-// Written with selects instead of branches; every candidate is the exact
-// expression of its reference case, so the selected value is bit-identical.
-__device__ __forceinline__ float div_rule(float v1, float v1l, float v2, float v2u, bool fr,
-                                          bool lr, bool fc, bool lc)
-{
-    const float a = fc ? v1 : (lc ? -v1l : (v1 - v1l));
-    const float t_first_row = a + v2;               // :70, corners :86-87
-    const float t_last_row = a - v2u;               // :71, corners :88-89
-    const float t_edge_col = (a + v2) - v2u;        // :81-82
-    const float t_body = a + (v2 - v2u);            // :59
-    return fr ? t_first_row : (lr ? t_last_row : ((fc || lc) ? t_edge_col : t_body));
-}
-
-// Thresholding operator TH, tvl1.cpp:263-287, branch-free: the three
-// reference cases multiply the warped gradient by l_t, -l_t or -rho/grad,
-// and the `grad < 1e-10` case stores +0 exactly as the reference does.
-__device__ __forceinline__ void th_step(float rc, float gx, float gy, float u1, float u2,
-                                        float l_t, float& v1, float& v2)
-{
-    const float grad = (gx * gx) + (gy * gy);
-    const float rho = rc + ((gx * u1) + (gy * u2));
-    const float fi = -rho / grad;
-    const bool c1 = rho >= -l_t * grad;
-    const bool c2 = rho > l_t * grad;
-    const bool c3 = grad < 1E-10f;
-    const bool zero = c1 && !c2 && c3;
-    const float k = c1 ? (c2 ? -l_t : fi) : l_t;
-    const float d1 = zero ? 0.0f : k * gx;
-    const float d2 = zero ? 0.0f : k * gy;
-    v1 = u1 + d1;
-    v2 = u2 + d2;
-}
-
-__device__ __forceinline__ double wave_sum_f64(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// One workgroup = 4 waves; the workgroup's extended tile is 64 columns (one
-// per lane) by 4*R rows (R consecutive rows per wave, held in registers).
-// Horizontal neighbours come from lane shuffles, vertical ones from the same
-// lane's registers or, across waves, from one LDS row.  A tile computes K
-// iterations; values within K pixels of the tile edge go stale one pixel per
-// iteration, so only the central (64-2K) x (4R-2K) pixels are stored.
-//
-// Stopping rule, evaluated by the last workgroup to finish: the warping stops
-// after the first iteration whose mean |u^n - u^(n-1)|^2 is not above
-// epsilon^2, or after max_iter iterations (tvl1.cpp:256-258).  When that
-// first iteration falls inside the launch before its last iteration, the
-// launch's output is discarded and the next launch re-runs exactly that many
-// iterations from the same input buffer.
-template <int K, int R>
-__global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
-{
-    constexpr int NW = 4;
-    constexpr int TW = 64 - 2 * K;
-    constexpr int TH = NW * R - 2 * K;
-    __shared__ float sh_a[NW][64], sh_b[NW][64], sh_c[NW][64], sh_d[NW][64];
-    __shared__ double sh_err[NW][K];
-    __shared__ double sh_tot[K];
-    __shared__ int sh_last;
-
-    TvCtl* ctl = a.ctl;
-    if (ctl->stop) return;
-    const int cur = ctl->cur[a.level];
-    const int redo = ctl->redo;
-    const int n0 = ctl->n;
-    int kk = redo ? redo : a.max_iter - n0;
-    if (kk > K) kk = K;
-    if (kk <= 0) return;
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
-    const int nx = a.nx, ny = a.ny;
-    const int x = (int)blockIdx.x * TW - K + lane;
-    const int ybase = (int)blockIdx.y * TH - K + wave * R;
-    const int oy0 = (int)blockIdx.y * TH;
-    const bool xin = x >= 0 && x < nx;
-    const bool xout = xin && lane >= K && lane < K + TW;
-    const bool fc = x == 0, lc = x == nx - 1;
-    // waves holding no first/last image column take the body formula of the
-    // divergence on every row away from the first/last image row
-    const bool edge_cols = __any(fc || lc);
-
-    const float* uin = a.u[cur];
-    const float* pin = a.p[cur];
-    const size_t plane = (size_t)a.plane;
-
-    float u1[R], u2[R], p11[R], p12[R], p21[R], p22[R], gx[R], gy[R], rc[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int y = ybase + r;
-        if (xin && y >= 0 && y < ny) {
-            const size_t idx = (size_t)y * nx + x;
-            u1[r] = uin[idx];
-            u2[r] = uin[plane + idx];
-            p11[r] = pin[idx];
-            p12[r] = pin[plane + idx];
-            p21[r] = pin[2 * plane + idx];
-            p22[r] = pin[3 * plane + idx];
-            gx[r] = a.wx[idx];
-            gy[r] = a.wy[idx];
-            rc[r] = a.rhoc[idx];
-        } else {
-            u1[r] = u2[r] = p11[r] = p12[r] = p21[r] = p22[r] = 0.0f;
-            gx[r] = gy[r] = rc[r] = 0.0f;
-        }
-    }
-
-    double acc[K];
-#pragma unroll
-    for (int t = 0; t < K; t++) acc[t] = 0.0;
-
-    const float l_t = a.l_t, theta = a.theta, taut = a.taut;
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        if (t < kk) {
-            // ---- u update: TH, divergence of p, u = v + theta*div p ----
-            sh_a[wave][lane] = p12[R - 1];
-            sh_b[wave][lane] = p22[R - 1];
-            __syncthreads();
-            const float up12 = wave ? sh_a[wave - 1][lane] : p12[0];
-            const float up22 = wave ? sh_b[wave - 1][lane] : p22[0];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int y = ybase + r;
-                const float p11l = from_left(p11[r]);
-                const float p21l = from_left(p21[r]);
-                const float p12u = r ? p12[r - 1] : up12;
-                const float p22u = r ? p22[r - 1] : up22;
-                const bool fr = y == 0, lr = y == ny - 1;
-                float div1, div2;
-                if (!edge_cols && !fr && !lr) {   // uniform branch: interior rows of interior waves
-                    div1 = (p11[r] - p11l) + (p12[r] - p12u);
-                    div2 = (p21[r] - p21l) + (p22[r] - p22u);
-                } else {
-                    div1 = div_rule(p11[r], p11l, p12[r], p12u, fr, lr, fc, lc);
-                    div2 = div_rule(p21[r], p21l, p22[r], p22u, fr, lr, fc, lc);
-                }
-                float v1, v2;
-                th_step(rc[r], gx[r], gy[r], u1[r], u2[r], l_t, v1, v2);
-                const float nu1 = v1 + theta * div1;
-                const float nu2 = v2 + theta * div2;
-                const float e1 = nu1 - u1[r], e2 = nu2 - u2[r];
-                const float e = e1 * e1 + e2 * e2;
-                const bool mine = xout && y >= oy0 && y < oy0 + TH && y < ny;
-                acc[t] += mine ? (double)e : 0.0;
-                u1[r] = nu1;
-                u2[r] = nu2;
-            }
-            // ---- p update: forward gradient of u, g, p = (p + taut*grad)/g ----
-            sh_c[wave][lane] = u1[0];
-            sh_d[wave][lane] = u2[0];
-            __syncthreads();
-            const float dn1 = wave < NW - 1 ? sh_c[wave + 1][lane] : u1[R - 1];
-            const float dn2 = wave < NW - 1 ? sh_d[wave + 1][lane] : u2[R - 1];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int y = ybase + r;
-                const float u1r = from_right(u1[r]);
-                const float u2r = from_right(u2[r]);
-                const float u1d = r < R - 1 ? u1[r + 1] : dn1;
-                const float u2d = r < R - 1 ? u2[r + 1] : dn2;
-                const bool xl = x < nx - 1, yl = y < ny - 1;
-                const float fx1 = xl ? u1r - u1[r] : 0.0f;
-                const float fy1 = yl ? u1d - u1[r] : 0.0f;
-                const float fx2 = xl ? u2r - u2[r] : 0.0f;
-                const float fy2 = yl ? u2d - u2[r] : 0.0f;
-                const float g1 = 1.0f + taut * hypot_ref(fx1, fy1);
-                const float g2 = 1.0f + taut * hypot_ref(fx2, fy2);
-                p11[r] = (p11[r] + taut * fx1) / g1;
-                p12[r] = (p12[r] + taut * fy1) / g1;
-                p21[r] = (p21[r] + taut * fx2) / g2;
-                p22[r] = (p22[r] + taut * fy2) / g2;
-            }
-        }
-    }
-
-    // ---- store the central tile into the other buffer ----
-    float* uout = a.u[cur ^ 1];
-    float* pout = a.p[cur ^ 1];
-    if (xout) {
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int y = ybase + r;
-            if (y >= oy0 && y < oy0 + TH && y < ny) {
-                const size_t idx = (size_t)y * nx + x;
-                uout[idx] = u1[r];
-                uout[plane + idx] = u2[r];
-                pout[idx] = p11[r];
-                pout[plane + idx] = p12[r];
-                pout[2 * plane + idx] = p21[r];
-                pout[3 * plane + idx] = p22[r];
-            }
-        }
-    }
-
-    // ---- per-workgroup error partials, fixed reduction order ----
-    const unsigned nblocks = gridDim.x * gridDim.y;
-    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        const double v = wave_sum_f64(acc[t]);
-        if (lane == 0) sh_err[wave][t] = v;
-    }
-    __syncthreads();
-    // The partials are published with device-scope atomic stores, which are
-    // written through to the point of coherence of all XCDs, and the wave
-    // waits for their completion (s_waitcnt vmcnt(0)) before the counter
-    // increment; the last workgroup reads them with device-scope atomic
-    // loads.  A device-scope release fence would instead write back the
-    // XCD's whole L2 at the end of every workgroup.
-    if (threadIdx.x < (unsigned)kk)
-        __hip_atomic_store(a.partials + (size_t)threadIdx.x * nblocks + bid,
-                           ((sh_err[0][threadIdx.x] + sh_err[1][threadIdx.x]) +
-                            sh_err[2][threadIdx.x]) + sh_err[3][threadIdx.x],
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-    // ---- the last workgroup reduces the partials and updates the control ----
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (threadIdx.x == 0) sh_last = atomicAdd(&ctl->counter, 1u) == nblocks - 1;
-    __syncthreads();
-    if (!sh_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int t = 0; t < kk; t++) {
-        double s = 0.0;
-        const double* col = a.partials + (size_t)t * nblocks;
-        for (unsigned j = threadIdx.x; j < nblocks; j += blockDim.x)
-            s += __hip_atomic_load(col + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum_f64(s);
-        if (lane == 0) sh_err[wave][0] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) sh_tot[t] = ((sh_err[0][0] + sh_err[1][0]) + sh_err[2][0]) + sh_err[3][0];
-        __syncthreads();
-    }
-    if (a.err_log && threadIdx.x < (unsigned)kk) a.err_log[n0 + threadIdx.x] = sh_tot[threadIdx.x];
-    if (threadIdx.x == 0) {
-        int n = n0, stop = 0, nredo = 0, flip = 0;
-        if (redo) {
-            n = n0 + redo;
-            stop = 1;
-            flip = 1;
-        } else {
-            int st = -1;
-            for (int t = 0; t < kk; t++) {
-                const double err = sh_tot[t] / (double)a.plane;
-                if (!(err > a.eps2)) { st = t; break; }
-            }
-            if (st == kk - 1 || (st < 0 && n0 + kk >= a.max_iter)) {
-                n = n0 + kk;
-                stop = 1;
-                flip = 1;
-            } else if (st >= 0) {
-                nredo = st + 1;
-            } else {
-                n = n0 + kk;
-                flip = 1;
-            }
-        }
-        ctl->n = n;
-        ctl->redo = nredo;
-        ctl->stop = stop;
-        if (flip) ctl->cur[a.level] = cur ^ 1;
-        if (stop) *a.iters_slot = n;
-        ctl->counter = 0;
-        ctl->launches = ctl->launches + 1;
-        const unsigned wl = ctl->wlaunches + 1;
-        ctl->wlaunches = wl;
-        const unsigned long long word = ((unsigned long long)a.epoch << 32) |
-                                        ((unsigned long long)stop << 31) |
-                                        (unsigned long long)(wl & 0x7FFFFFFFu);
-        // the host only reads this word itself: no ordering with other data
-        if (a.host_status)
-            __hip_atomic_store(a.host_status, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
 }
 
 __global__ __launch_bounds__(256) void k_hypot(const float* __restrict__ a,
@@ -619,47 +333,66 @@ __global__ __launch_bounds__(256) void k_hypot(const float* __restrict__ a,
         o[i] = hypot_ref(a[i], b[i]);
 }
 
+// TV-L1 energy of a flow at one scale (no reference counterpart; it decides
+// between a predicted initial flow and zero):
+//   E(u) = lambda * sum_x |I1(x + u(x)) - I0(x)| + sum_x (|grad u1| + |grad u2|)
+// I1 is sampled with the Keys bicubic and a clamped border, the gradient is
+// the forward difference (zero across the last column / row), |.| of a
+// gradient is hypot.  Per-pixel terms are float; the two sums are double,
+// reduced per block in a fixed tree, partials [2*block] = (data, tv).
+__global__ __launch_bounds__(256) void k_energy(const float* __restrict__ I0,
+                                                const float* __restrict__ I1,
+                                                const float* __restrict__ u1,
+                                                const float* __restrict__ u2, int nx, int ny,
+                                                double* __restrict__ part)
+{
+    __shared__ double sd[256], st[256];
+    const long long n = (long long)nx * ny;
+    double data = 0.0, tv = 0.0;
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int y = (int)(i / nx), x = (int)(i - (long long)y * nx);
+        const float a = u1[i], b = u2[i];
+        const float w = bicubic_at(I1, (float)x + a, (float)y + b, nx, ny, false);
+        data += (double)fabsf(w - I0[i]);
+        const float ax = x < nx - 1 ? u1[i + 1] - a : 0.0f;
+        const float ay = y < ny - 1 ? u1[i + nx] - a : 0.0f;
+        const float bx = x < nx - 1 ? u2[i + 1] - b : 0.0f;
+        const float by = y < ny - 1 ? u2[i + nx] - b : 0.0f;
+        tv += (double)hypot_ref(ax, ay) + (double)hypot_ref(bx, by);
+    }
+    sd[threadIdx.x] = data;
+    st[threadIdx.x] = tv;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+            st[threadIdx.x] += st[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sd[0];
+        part[2 * blockIdx.x + 1] = st[0];
+    }
+}
+
+__global__ void k_energy_final(const double* __restrict__ part, int nblocks, double lambda,
+                               double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double d = 0.0, t = 0.0;
+    for (int b = 0; b < nblocks; b++) {
+        d += part[2 * b];
+        t += part[2 * b + 1];
+    }
+    out[0] = lambda * d + t;
+    out[1] = d;
+    out[2] = t;
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-
-bool iter_config_ok(int K, int R)
-{
-    if (!(K == 1 || K == 2 || K == 4 || K == 8)) return false;
-    if (!(R == 2 || R == 4 || R == 8)) return false;
-    return 4 * R - 2 * K >= 4;
-}
-
-void iter_grid(int K, int R, int nx, int ny, unsigned* gx, unsigned* gy)
-{
-    *gx = cdiv((size_t)nx, (size_t)(64 - 2 * K));
-    *gy = cdiv((size_t)ny, (size_t)(4 * R - 2 * K));
-}
-
-template <int K, int R>
-static hipError_t launch_kr(const IterArgs& a, hipStream_t s)
-{
-    unsigned gx, gy;
-    iter_grid(K, R, a.nx, a.ny, &gx, &gy);
-    hipLaunchKernelGGL((k_tvl1_iter<K, R>), dim3(gx, gy), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-int launch_iter(int K, int R, const IterArgs& a, hipStream_t s)
-{
-    hipError_t e = hipErrorInvalidValue;
-#define OF_CASE(k, r) \
-    if (K == k && R == r) e = launch_kr<k, r>(a, s);
-    OF_CASE(1, 2) OF_CASE(2, 2)
-    OF_CASE(1, 4) OF_CASE(2, 4) OF_CASE(4, 4)
-    OF_CASE(1, 8) OF_CASE(2, 8) OF_CASE(4, 8) OF_CASE(8, 8)
-#undef OF_CASE
-    if (e != hipSuccess) {
-        record_hip_error(e, "k_tvl1_iter launch", __FILE__, __LINE__);
-        return OF_ERR_HIP;
-    }
-    return OF_OK;
-}
 
 static inline dim3 grid2d(int nx, int ny) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
 static inline unsigned grid1d(size_t n) { unsigned g = cdiv(n, 256); return g > 4096 ? 4096 : (g ? g : 1); }
@@ -780,10 +513,10 @@ int launch_warp_setup(const float* I0, const float* I1, const float* I1x, const 
 }
 
 int launch_copy_flow(const float* ub0, const float* ub1, float* out, size_t n2, const TvCtl* ctl,
-                     hipStream_t s)
+                     int level, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_flow, dim3(grid1d(n2)), dim3(256), 0, s, ub0, ub1, out,
-                       (unsigned)n2, ctl);
+                       (unsigned)n2, ctl, level);
     OF_LAUNCH_CHECK("k_copy_flow");
     return OF_OK;
 }
@@ -792,6 +525,21 @@ int launch_hypot(const float* a, const float* b, float* o, size_t n, hipStream_t
 {
     hipLaunchKernelGGL(k_hypot, dim3(grid1d(n)), dim3(256), 0, s, a, b, o, (unsigned)n);
     OF_LAUNCH_CHECK("k_hypot");
+    return OF_OK;
+}
+
+// out[0] = E, out[1] = data sum, out[2] = TV sum; `partials` holds
+// 2*max_partials doubles.
+int launch_energy(const float* I0, const float* I1, const float* u1, const float* u2, int nx,
+                  int ny, float lambda, double* partials, int max_partials, double* out,
+                  hipStream_t s)
+{
+    const unsigned blocks = std::min(grid1d((size_t)nx * ny), (unsigned)max_partials);
+    hipLaunchKernelGGL(k_energy, dim3(blocks), dim3(256), 0, s, I0, I1, u1, u2, nx, ny, partials);
+    OF_LAUNCH_CHECK("k_energy");
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, s, partials, (int)blocks,
+                       (double)lambda, out);
+    OF_LAUNCH_CHECK("k_energy_final");
     return OF_OK;
 }
 

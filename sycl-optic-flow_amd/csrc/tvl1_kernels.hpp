@@ -23,6 +23,10 @@ struct TvCtl {
     unsigned wlaunches;         // the same, counted within the current warping
     unsigned long long maxkey[2];   // (|x| bits << 32) | (~index), per image
     unsigned long long minkey[2];   // (|x| bits << 32) | index, per image
+    // grid barrier of the persistent warping kernel
+    unsigned bar_count;
+    unsigned bar_gen;
+    int error;                      // set if a barrier wait ran past its bound
 };
 
 struct IterArgs {
@@ -45,9 +49,13 @@ struct IterArgs {
     double* err_log;    // if set: per-iteration error sums, indexed by iteration
 };
 
-// Launch helpers implemented in tvl1_kernels.hip.
+// Launch helpers implemented in tvl1_iter.hip.
 int launch_iter(int K, int R, const IterArgs& a, hipStream_t s);
 bool iter_config_ok(int K, int R);
 void iter_grid(int K, int R, int nx, int ny, unsigned* gx, unsigned* gy);
+// Whole warping in one cooperative launch; `nblocks` = 0 picks the largest
+// co-resident grid not above the tile count.  Returns OF_ERR_UNSUPPORTED if
+// the device cannot launch cooperatively.
+int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s);
 
 }  // namespace amdflow

@@ -32,7 +32,10 @@ int launch_warp_setup(const float* I0, const float* I1, const float* I1x, const 
                       const float* ub0, const float* ub1, float* wx, float* wy, float* rhoc,
                       int nx, int ny, TvCtl* ctl, int level, hipStream_t s);
 int launch_copy_flow(const float* ub0, const float* ub1, float* out, size_t n2, const TvCtl* ctl,
-                     hipStream_t s);
+                     int level, hipStream_t s);
+int launch_energy(const float* I0, const float* I1, const float* u1, const float* u2, int nx,
+                  int ny, float lambda, double* partials, int max_partials, double* out,
+                  hipStream_t s);
 
 int launch_hypot(const float* a, const float* b, float* o, size_t n, hipStream_t s);
 

@@ -178,15 +178,33 @@ def test_tvl1_iteration_cap_reached():
     assert np.array_equal(u, uo)
 
 
-@pytest.mark.parametrize("K,R", [(1, 2), (4, 4), (8, 8)])
-def test_tvl1_launch_shapes_agree(K, R):
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("K,R", [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8),
+                                 (8, 8)])
+def test_tvl1_launch_shapes_agree(K, R, persistent):
     I0, I1, _, _ = synth.make_pair(240, 180, 5.0, 13, bumps=True)
     q0, q1 = synth.quantize(I0), synth.quantize(I1)
     with TV_L1(240, 180) as tv:
-        tv.set_launch_rule(0, K, R)
+        tv.set_launch_rule(0, K, R, persistent)
         tv.runDualTVL1Multiscale(q0, q1)
         u, it = tv.getU(), tv.iterations()
     uo, ito = O.tvl1(q0, q1)
+    assert np.array_equal(it, ito)
+    assert np.array_equal(u, uo)
+
+
+@pytest.mark.parametrize("K,R", [(2, 8), (4, 4)])
+def test_tvl1_persistent_everywhere_1080p(K, R):
+    """The whole 1920x1080 solve with persistent warpings at every scale
+    (at 1920x1080 each workgroup loops over several tiles)."""
+    cfg = synth.CONFIGS["tvl1_1920x1080"]
+    I0, I1, _, _ = synth.make_pair(cfg["width"], cfg["height"], cfg["cap"], 2024, bumps=True)
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    with TV_L1(cfg["width"], cfg["height"], nscales=5) as tv:
+        tv.set_launch_rule(0, K, R, True)
+        tv.runDualTVL1Multiscale(q0, q1)
+        u, it = tv.getU(), tv.iterations()
+    uo, ito = O.tvl1(q0, q1, O.default_params(nscales=5))
     assert np.array_equal(it, ito)
     assert np.array_equal(u, uo)
 

@@ -378,3 +378,56 @@ def test_lk_pyramidal_recovers_translation():
     inner = (slice(30, -30), slice(30, -30))
     assert abs(float(np.median(u[inner])) - 3.2) < 0.05
     assert abs(float(np.median(v[inner])) + 1.7) < 0.05
+
+
+# ------------------------------------------------------------ TV-L1 energy
+
+
+def test_energy_of_zero_flow_is_lambda_l1_of_difference():
+    rng = np.random.default_rng(3)
+    I0 = (rng.random((23, 31)) * 255).astype(np.float32)
+    I1 = (rng.random((23, 31)) * 255).astype(np.float32)
+    u = np.zeros(2 * I0.size, np.float32)
+    e, d, t = O.tvl1_energy(I0, I1, u, 0.15)
+    ref = np.abs(I1 - I0).astype(np.float64).sum()
+    assert t == 0.0 and abs(d - ref) <= 1e-12 * ref
+    assert abs(e - float(np.float32(0.15)) * ref) <= 1e-12 * e   # lambda is a float parameter
+
+
+@pytest.mark.parametrize("dx,dy", [(2, -1), (-3, 0), (0, 4), (40, -50)])
+def test_energy_of_integer_shift(dx, dy):
+    """A constant integer flow samples I1 exactly at the clamped shifted
+    pixel (the Keys kernel interpolates; all taps clamp to the edge value
+    outside), and its gradient is zero."""
+    rng = np.random.default_rng(100 + dx * 7 + dy)
+    h, w = 19, 26
+    I0 = (rng.random((h, w)) * 255).astype(np.float32)
+    I1 = (rng.random((h, w)) * 255).astype(np.float32)
+    u = np.concatenate([np.full(h * w, dx, np.float32), np.full(h * w, dy, np.float32)])
+    e, d, t = O.tvl1_energy(I0, I1, u, 0.15)
+    yy, xx = np.mgrid[0:h, 0:w]
+    I1s = I1[np.clip(yy + dy, 0, h - 1), np.clip(xx + dx, 0, w - 1)]
+    ref = np.abs(I1s - I0).astype(np.float64).sum()
+    assert t == 0.0 and abs(d - ref) <= 1e-12 * ref
+
+
+def test_energy_tv_of_linear_ramp_flow():
+    """u1 = a*x + b*y, u2 = 0 on a grid: |grad u1| = hypot(a, b) except on
+    the last row / column, where one difference is zero."""
+    h, w = 9, 12
+    a, b = 0.25, -0.5
+    yy, xx = np.mgrid[0:h, 0:w]
+    u = np.concatenate([(a * xx + b * yy).ravel(), np.zeros(h * w)]).astype(np.float32)
+    I = np.zeros((h, w), np.float32)
+    _, _, t = O.tvl1_energy(I, I, u, 0.15)
+    hyp = float(np.hypot(np.float32(a), np.float32(b)))   # per-pixel terms are float
+    ref = (h - 1) * (w - 1) * hyp + (h - 1) * abs(b) + (w - 1) * abs(a)
+    assert abs(t - ref) <= 1e-9
+
+
+def test_tvl1_init_none_is_plain_solve():
+    I0, I1, _, _ = synth.make_pair(64, 48, 2.0, 3)
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    a, ia = O.tvl1(q0, q1)
+    b, ib = O.tvl1_init(q0, q1, None)
+    assert np.array_equal(a, b) and np.array_equal(ia, ib)
