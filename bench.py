@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--batch-pairs", type=int, default=64,
                     help="pairs of the batched mixed-size measurement (0 = skip)")
     ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--no-other", action="store_true", help="skip the HS and LK measurements")
+    ap.add_argument("--warm-pairs", type=int, default=16,
+                    help="pairs of the warm-start measurement (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -107,6 +110,129 @@ def batched(npairs: int, streams: int, reps: int):
             "data": "synthetic, 2 generated pairs per size cycled through the batch"}
 
 
+def _cores() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+
+def _timed(fn, reps: int):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def _cpu_timed(fn, budget_s: float):
+    t0 = time.perf_counter()
+    runs = 0
+    while True:
+        fn()
+        runs += 1
+        if time.perf_counter() - t0 >= budget_s or runs >= 20:
+            break
+    return (time.perf_counter() - t0) / runs, runs
+
+
+def other_paths(cpu_budget: float):
+    """Horn-Schunck 640x480 (1 level, 1 warping, 100 Jacobi sweeps, alpha 15)
+    and pyramidal Lucas-Kanade 1280x720 (3 levels, zoom 0.5, sigma 1, 7x7,
+    5 iterations, det > 1e-6) on float frames in HBM, with their oracle
+    timed on the host; algorithmic bytes: HS 28 B per pixel-sweep + 52 B per
+    pixel-warping, LK 44 B per pixel-iteration + 24 B per pixel-level."""
+    import numpy as np
+    import torch
+
+    from amdflow import HornSchunck, lk_pyramidal_device, synth
+    out = {}
+    cfg = synth.CONFIGS["hs_640x480"]
+    W, H = cfg["width"], cfg["height"]
+    I0, I1, _, _ = synth.make_pair(W, H, cfg["cap"], SEED + 2, bumps=False)
+    d0, d1 = torch.from_numpy(I0.reshape(-1)).cuda(), torch.from_numpy(I1.reshape(-1)).cuda()
+    du = torch.empty(W * H, device="cuda")
+    dv = torch.empty(W * H, device="cuda")
+    hs = HornSchunck(1, W, W, H)
+    sp = torch.cuda.current_stream().cuda_stream
+    t = _timed(lambda: hs.compute_device(d0.data_ptr(), d1.data_ptr(), du.data_ptr(), dv.data_ptr(),
+                                         15.0, 1, 1, 100, sp), 50)
+    alg = 28.0 * 100 * W * H + 52.0 * W * H
+    rec = {"workload": "hs_640x480", "pairs_per_s": round(1 / t, 1), "ms_per_pair": round(t * 1e3, 4),
+           "algorithmic_GBps": round(alg / t / 1e9, 1)}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    tc, runs = _cpu_timed(lambda: O.hs_compute(I0, I1, 15.0, 1, 1, 100), cpu_budget)
+    rec["cpu_baseline"] = {"value": round(1 / tc, 2), "unit": "frame-pairs/s", "kind": "port",
+                           "cores": _cores(), "sample": f"{runs} solve(s), OpenMP"}
+    u_o, v_o = O.hs_compute(I0, I1, 15.0, 1, 1, 100)
+    rec["bitwise_equal_to_oracle"] = bool(np.array_equal(du.cpu().numpy(), u_o.reshape(-1))
+                                          and np.array_equal(dv.cpu().numpy(), v_o.reshape(-1)))
+    out["hs"] = rec
+
+    cfg = synth.CONFIGS["lk_1280x720"]
+    W, H = cfg["width"], cfg["height"]
+    I0, I1, _, _ = synth.make_pair(W, H, cfg["cap"], SEED + 3, bumps=False)
+    d0, d1 = torch.from_numpy(I0.reshape(-1)).cuda(), torch.from_numpy(I1.reshape(-1)).cuda()
+    du = torch.empty(W * H, device="cuda")
+    dv = torch.empty(W * H, device="cuda")
+    t = _timed(lambda: lk_pyramidal_device(d0.data_ptr(), d1.data_ptr(), W, H, du.data_ptr(),
+                                           dv.data_ptr(), 3, 0.5, 1.0, 7, 5, 1e-6, sp), 50)
+    npx = W * H + (W // 2) * (H // 2) + (W // 4) * (H // 4)
+    alg = 44.0 * 5 * npx + 24.0 * npx
+    rec = {"workload": "lk_1280x720", "pairs_per_s": round(1 / t, 1), "ms_per_pair": round(t * 1e3, 4),
+           "algorithmic_GBps": round(alg / t / 1e9, 1)}
+    tc, runs = _cpu_timed(lambda: O.lk_pyr(I0, I1), cpu_budget)
+    rec["cpu_baseline"] = {"value": round(1 / tc, 2), "unit": "frame-pairs/s", "kind": "port",
+                           "cores": _cores(), "sample": f"{runs} solve(s), OpenMP"}
+    u_o, v_o = O.lk_pyr(I0, I1)
+    rec["bitwise_equal_to_oracle"] = bool(np.array_equal(du.cpu().numpy(), u_o.reshape(-1))
+                                          and np.array_equal(dv.cpu().numpy(), v_o.reshape(-1)))
+    out["lk"] = rec
+    return out
+
+
+def cnn_mfma(net, batch: int, h: int, w: int, reps: int = 20):
+    """Forward-pass throughput of the warm-start network on `batch` coarsest
+    scale inputs, timed with HIP events on the stream it runs on."""
+    import torch
+    s = torch.cuda.Stream()
+    x = torch.randn(batch * h * w * 5, device="cuda")
+    y = torch.empty(batch * h * w * 2, device="cuda")
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            net.forward(x.data_ptr(), batch, h, w, y.data_ptr(), s.cuda_stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            net.forward(x.data_ptr(), batch, h, w, y.data_ptr(), s.cuda_stream)
+        e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    macs = sum(9 * ci * co for ci, co in [(5, 32)] + [(32, 32)] * net.hidden + [(32, 2)])
+    flops = 2.0 * macs * batch * h * w
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(tflops / 2500.0, 5), "flops_per_forward": flops,
+            "ms_per_forward": round(ms, 4), "batch": batch, "shape": [h, w, 5], "dtype": "bf16"}
+
+
+def warm_start(npairs: int):
+    """Plain vs warm-started TV-L1 on `npairs` held-out 1920x1080 pairs (the
+    network trained on other seeds), EPE against the converged plain solve."""
+    from amdflow import cnn
+    from amdflow.warmstart import evaluate
+    if not os.path.exists(cnn.WEIGHTS):
+        return {"skipped": "no trained weights at " + os.path.relpath(cnn.WEIGHTS, ROOT)}
+    net = cnn.WarmStartNet.pretrained()
+    res = evaluate(net, npairs=npairs)
+    res["workload"] = f"tvl1_warm_1920x1080_batch{npairs}"
+    res["network"] = {"layers": f"conv3x3 5-32, {net.hidden}x 32-32, 32-2", "params": net.num_params}
+    res["mfma"] = cnn_mfma(net, npairs, 68, 120)
+    return res
+
+
 def cpu_baseline(q0, q1, budget_s: float):
     """Times the oracle on the same pair; returns (record, flow, iterations)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -122,7 +248,7 @@ def cpu_baseline(q0, q1, budget_s: float):
         if time.perf_counter() - t0 >= budget_s or runs >= 50:
             break
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    cores = _cores()
     rec = {"value": runs / dt, "unit": "frame-pairs/s", "cores": cores, "kind": "port",
            "sample": f"{runs} solve(s) of the same 1920x1080 u8 pair, OpenMP, {dt:.1f} s"}
     return rec, u, it
@@ -194,7 +320,8 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-        "kernel": f"k_tvl1_iter<{dom['iters_per_launch']},{dom['rows_per_wave']}>",
+        "kernel": ("k_tvl1_persist" if dom["persistent"] else "k_tvl1_iter")
+                  + f"<{dom['iters_per_launch']},{dom['rows_per_wave']}>",
         "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(avg_launch_s * 1e6, 2),
         "launches": dom["launches"],
         "kernel_share_of_step": round(dom["milliseconds"] / max(1, args.profile_steps)
@@ -228,6 +355,10 @@ def main():
     }
     if args.batch_pairs > 0:
         result["batched"] = batched(args.batch_pairs, args.batch_streams, reps=3)
+    if args.warm_pairs > 0:
+        result["warm_start"] = warm_start(args.warm_pairs)
+    if not args.no_other:
+        result["other_paths"] = other_paths(3.0 if not args.no_cpu else 0.0)
     if not args.no_cpu:
         rec, uo, ito = cpu_baseline(q0, q1, args.cpu_seconds)
         result["cpu_baseline"] = rec

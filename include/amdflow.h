@@ -8,7 +8,8 @@
  * are synchronous: results are in host memory when they return, as in the
  * reference (src/tvl1/cuda/tvl1.cu:181-182, src/hs/cuda/flow.cu:206-207).
  * The *_device variants take device pointers and a HIP stream (passed as
- * void*) and return once the work is complete on that stream.
+ * void*; NULL is the HIP null stream) and return once the work is complete
+ * on that stream.
  */
 #ifndef AMDFLOW_H
 #define AMDFLOW_H
@@ -92,8 +93,8 @@ const float *of_tvl1_flow(of_tvl1 *h);
 /* Device-resident variant used by throughput measurements and by callers
  * that keep frames in HBM: dI0/dI1 are device pointers to width*height
  * bytes, dU a device pointer to 2*width*height floats (planar [u1|u2]).
- * `stream` is a hipStream_t (NULL = the handle's own stream).  Returns after
- * the solve has completed on that stream. */
+ * `stream` is a hipStream_t (NULL = the HIP null stream).  Returns after the
+ * solve has completed on that stream. */
 int of_tvl1_run_device_u8(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, float *dU,
                           void *stream);
 int of_tvl1_run_device_f32(of_tvl1 *h, const float *dI0, const float *dI1, float *dU,
@@ -270,6 +271,22 @@ int of_tvl1_warm_run_device_u8(of_tvl1 *tv, of_cnn *net, const uint8_t *dI0, con
 int of_synth_pair(int width, int height, float cap, float noise, unsigned long long seed,
                   const float *params, int nbumps, uint8_t *dI0, uint8_t *dI1, float *dflow,
                   float *dscratch, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Output formats of a planar flow [u | v] (width*height floats each).
+ * ---------------------------------------------------------------------- */
+/* App::flowToColor (src/tvl1/sequential/main.cpp:93-126): hue = atan2(v,u)
+ * in degrees / 2, value = min(10*|flow|, 255), each rounded to 8 bits with
+ * saturation, saturation 255, HSV -> BGR; 3 bytes per pixel, row-major.
+ * Device variant asynchronous on `stream`; host variant synchronous. */
+int of_flow_to_color_device(const float *dU, int width, int height, uint8_t *dBGR, void *stream);
+int of_flow_to_color(const float *U, int width, int height, uint8_t *bgr);
+/* Middlebury .flo (no reference counterpart): float tag 202021.25, int32
+ * width, int32 height, then (u, v) float pairs row by row, little-endian.
+ * of_flo_read stores the size and, when U is not NULL, the planar flow
+ * (cap = capacity of U in floats). */
+int of_flo_write(const char *path, const float *U, int width, int height);
+int of_flo_read(const char *path, float *U, long long cap, int *width, int *height);
 
 /* ------------------------------------------------------------------------
  * Horn-Schunck.  Replaces initFlow / ComputeFlow / deleteFlow_mem,

@@ -1,8 +1,8 @@
 """Persisted launch shapes found by the TV-L1 autotuner (of_tvl1_autotune).
 
-The table maps a workload key "WxH/s<nscales>/w<warps>/<arch>" to the
-(iterations per launch, rows per wave) pair of every pyramid scale, finest
-first, plus the per-shape timings the choice was made from.
+The table maps a workload key "WxH/s<nscales>/w<warps>/<arch>" to the launch
+variant (iterations per launch, rows per wave, persistent) of every pyramid
+scale, finest first, plus the per-variant timings the choice was made from.
 """
 from __future__ import annotations
 

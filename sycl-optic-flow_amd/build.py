@@ -32,6 +32,7 @@ SOURCES = [
     "lk.hip",
     "cnn.hip",
     "synth.hip",
+    "output.cpp",
 ]
 
 ARCH = os.environ.get("AMDFLOW_ARCH", "gfx950")

@@ -32,7 +32,8 @@
 namespace amdflow {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ short to_bf16(float f)
@@ -44,12 +45,14 @@ __host__ __device__ __forceinline__ short to_bf16(float f)
     return (short)(u >> 16);
 }
 
-// 16x16x16 MFMA: lane l supplies row (l % 16) of A and column (l % 16) of B
-// for k = 4*(l/16) .. +3, and receives column (l % 16), rows 4*(l/16) .. +3
-// of the product (of_dbg_cnn_mfma checks this through k_conv).
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+// gfx950 16x16x32 MFMA: lane l supplies row (l % 16) of A and column
+// (l % 16) of B for k = 8*(l/16) .. +7, and receives column (l % 16), rows
+// 4*(l/16) .. +3 of the 16x16 product (of_dbg_cnn_mfma checks this through
+// k_conv).
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
 struct Pix {
@@ -59,36 +62,49 @@ struct Pix {
 __device__ __forceinline__ Pix pix_of(long long m, int H, int W)
 {
     Pix p;
-    const long long hw = (long long)H * W;
-    p.b = (int)(m / hw);
-    const int r = (int)(m - (long long)p.b * hw);
+    const int hw = H * W;
+    const int mi = (int)m;   // pixel counts stay below 2^31
+    p.b = mi / hw;
+    const int r = mi - p.b * hw;
     p.y = r / W;
     p.x = r - p.y * W;
     return p;
 }
 
-// Four consecutive GEMM-k entries k .. k+3 of pixel p's im2col row
+__device__ __forceinline__ void pix_next(Pix& p, int H, int W)
+{
+    if (++p.x == W) {
+        p.x = 0;
+        if (++p.y == H) {
+            p.y = 0;
+            p.b++;
+        }
+    }
+}
+
+// Eight consecutive GEMM-k entries k .. k+7 of pixel p's im2col row
 // (k = t*CX + c), gathered from an NHWC tensor with CX channels; `sgn` = -1
 // mirrors the tap offsets (data gradient).
 template <int CX>
-__device__ __forceinline__ bf16x4 gather4(const float* __restrict__ X, const Pix& p, bool valid,
-                                          int k, int K, int H, int W, int sgn)
+__device__ __forceinline__ s16x8 gather8(const float* __restrict__ X, const Pix& p, bool valid,
+                                         int k, int K, int H, int W, int sgn)
 {
-    bf16x4 r;
-    if constexpr (CX % 4 == 0) {
-        // the four entries share a tap: one 16-byte load
+    s16x8 r;
+    if constexpr (CX % 8 == 0) {
+        // the eight entries share a tap: two 16-byte loads
         const int t = k / CX, c = k - t * CX;
         const int yy = p.y + sgn * (t / 3 - 1), xx = p.x + sgn * (t % 3 - 1);
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (valid && k < K && yy >= 0 && yy < H && xx >= 0 && xx < W)
-            v = *reinterpret_cast<const float4*>(X + (((long long)p.b * H + yy) * W + xx) * CX + c);
-        r[0] = to_bf16(v.x);
-        r[1] = to_bf16(v.y);
-        r[2] = to_bf16(v.z);
-        r[3] = to_bf16(v.w);
+        float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+        if (valid && k < K && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const float4* q = reinterpret_cast<const float4*>(X + (((long long)p.b * H + yy) * W + xx) * CX + c);
+            v0 = q[0];
+            v1 = q[1];
+        }
+        r[0] = to_bf16(v0.x); r[1] = to_bf16(v0.y); r[2] = to_bf16(v0.z); r[3] = to_bf16(v0.w);
+        r[4] = to_bf16(v1.x); r[5] = to_bf16(v1.y); r[6] = to_bf16(v1.z); r[7] = to_bf16(v1.w);
     } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < 8; j++) {
             const int kk = k + j;
             const int t = kk / CX, c = kk - t * CX;
             const int yy = p.y + sgn * (t / 3 - 1), xx = p.x + sgn * (t % 3 - 1);
@@ -102,9 +118,10 @@ __device__ __forceinline__ bf16x4 gather4(const float* __restrict__ X, const Pix
 }
 
 // Implicit-GEMM 3x3 convolution.  A block is 4 waves; a wave owns 16
-// pixels and NT 16-wide tiles of output channels.  Wp: bf16 [NT*16][Kpad],
-// row n = output channel, zero beyond K and N.  Epilogue: + bias, ReLU,
-// and/or zero where mask[m][n] <= 0 (ReLU derivative of the layer below).
+// pixels and NT 16-wide tiles of output channels.  Wp: bf16 [NT*16][Kpad]
+// (Kpad a multiple of 32), row n = output channel, zero beyond K and N.
+// Epilogue: + bias, ReLU, and/or zero where mask[m][n] <= 0 (the ReLU
+// derivative of the layer below).
 template <int CX, int NT>
 __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ X, int B, int H, int W,
                                               const short* __restrict__ Wp, int Kpad, int N,
@@ -124,13 +141,13 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ X, int B
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < Kpad; k0 += 16) {
-        const bf16x4 a = gather4<CX>(X, p, valid, k0 + 4 * kg, K, H, W, sgn);
+    for (int k0 = 0; k0 < Kpad; k0 += 32) {
+        const s16x8 a = gather8<CX>(X, p, valid, k0 + 8 * kg, K, H, W, sgn);
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const bf16x4 b =
-                *reinterpret_cast<const bf16x4*>(Wp + (size_t)(t * 16 + row) * Kpad + k0 + 4 * kg);
-            acc[t] = mfma16(a, b, acc[t]);
+            const s16x8 b =
+                *reinterpret_cast<const s16x8*>(Wp + (size_t)(t * 16 + row) * Kpad + k0 + 8 * kg);
+            acc[t] = mfma32(a, b, acc[t]);
         }
     }
 #pragma unroll
@@ -152,8 +169,8 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ X, int B
 
 // Weight-gradient partials.  Block (kt, nt, z) forms the 16x16 tile
 // dW[16*nt ..][16*kt ..] over the pixels [z*chunk, (z+1)*chunk): the four
-// waves take interleaved 16-pixel steps (A = dY^T, B = im2col(X)) and are
-// then added in wave order.  part: [splits][Npad][Kpad].
+// waves take interleaved 32-pixel steps (A = dY^T, B = im2col(X), GEMM-k =
+// pixels) and are then added in wave order.  part: [splits][Npad][Kpad].
 template <int CX>
 __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY, int N,
                                                const float* __restrict__ X, int B, int H, int W,
@@ -178,16 +195,17 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY, int
     }
     const int n = n0 + row;   // A row
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (long long q0 = p_begin + wave * 16; q0 < p_end; q0 += 64) {
-        bf16x4 a, b;
+    for (long long q0 = p_begin + wave * 32; q0 < p_end; q0 += 128) {
+        s16x8 a, b;
+        const long long qf = q0 + 8 * kg;
+        Pix p = pix_of(qf < M ? qf : 0, H, W);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long long q = q0 + 4 * kg + j;
+        for (int j = 0; j < 8; j++) {
+            const long long q = qf + j;
             float va = 0.0f, vb = 0.0f;
             if (q < p_end) {
                 if (n < N) va = dY[q * N + n];
                 if (kvalid) {
-                    const Pix p = pix_of(q, H, W);
                     const int y = p.y + dy, x = p.x + dx;
                     if (y >= 0 && y < H && x >= 0 && x < W)
                         vb = X[(((long long)p.b * H + y) * W + x) * CX + c];
@@ -195,8 +213,9 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY, int
             }
             a[j] = to_bf16(va);
             b[j] = to_bf16(vb);
+            pix_next(p, H, W);
         }
-        acc = mfma16(a, b, acc);
+        acc = mfma32(a, b, acc);
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) sh[wave][4 * kg + j][row] = acc[j];   // [n][k]
@@ -359,6 +378,7 @@ __global__ __launch_bounds__(256) void k_to_planar(const float* __restrict__ Y, 
 }
 
 inline int pad16(int v) { return (v + 15) / 16 * 16; }
+inline int pad32(int v) { return (v + 31) / 32 * 32; }
 inline unsigned blocks256(long long n)
 {
     const long long g = (n + 255) / 256;
@@ -381,7 +401,7 @@ int launch_conv(const float* X, int B, int H, int W, int CX, const short* Wp, in
     const long long M = (long long)B * H * W;
     const unsigned blocks = (unsigned)((M + 63) / 64);
     const int NT = N <= 16 ? 1 : 2;
-    if (N > 32 || Kpad < 9 * CX || Kpad % 16) return OF_ERR_UNSUPPORTED;
+    if (N > 32 || Kpad < 9 * CX || Kpad % 32) return OF_ERR_UNSUPPORTED;
 #define CONV_CASE(cx, nt)                                                                          \
     if (CX == cx && NT == nt) {                                                                    \
         hipLaunchKernelGGL((k_conv<cx, nt>), dim3(blocks), dim3(256), 0, st, X, B, H, W, Wp, Kpad, \
@@ -621,10 +641,10 @@ int of_cnn_create(int hidden_layers, int channels, unsigned long long seed, of_c
         off += (size_t)L.cout * 9 * L.cin;
         L.boff = off;
         off += L.cout;
-        L.Kpad = pad16(9 * L.cin);
+        L.Kpad = pad32(9 * L.cin);
         L.Npad = pad16(L.cout);
         L.Cpad = pad16(L.cin);
-        L.Kbpad = pad16(9 * L.cout);
+        L.Kbpad = pad32(9 * L.cout);
         h->layers.push_back(L);
     }
     h->nparams = off;
@@ -801,7 +821,7 @@ int of_dbg_cnn_mfma(const float* A, const float* Bm, float* D)
     float* dX = nullptr;
     float* dY = nullptr;
     short* dW = nullptr;
-    const int Kpad = 144;
+    const int Kpad = 160;
     int st = dev_alloc(&dX, 256);
     if (!st) st = dev_alloc(&dY, 256);
     if (!st) st = dev_alloc(&dW, 16 * Kpad);

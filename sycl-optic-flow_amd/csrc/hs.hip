@@ -474,7 +474,7 @@ int of_hs_compute_device(of_hs* h, const float* dI0, const float* dI1, int width
                          float* du, float* dv, void* stream)
 {
     if (!h || !dI0 || !dI1 || !du || !dv) return OF_ERR_ARG;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     return hs_solve(h, dI0, dI1, width, height, stride, alpha, nLevels, nWarpIters, nSolverIters,
                     du, dv, st);
 }

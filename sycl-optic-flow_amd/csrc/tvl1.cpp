@@ -623,21 +623,21 @@ int of_tvl1_scale_size(const of_tvl1* h, int s, int* nx, int* ny)
 int of_tvl1_run_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, float* dU, void* stream)
 {
     if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     return solve(h, dI0, dI1, true, dU, st);
 }
 
 int of_tvl1_run_device_f32(of_tvl1* h, const float* dI0, const float* dI1, float* dU, void* stream)
 {
     if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     return solve(h, dI0, dI1, false, dU, st);
 }
 
 int of_tvl1_prepare_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, void* stream)
 {
     if (!h || !dI0 || !dI1) return OF_ERR_ARG;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     return prepare(h, dI0, dI1, true, st);
 }
 
@@ -655,7 +655,7 @@ int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int warps, int max_i
 {
     if (!h || !dU || warps < 0 || warps > h->prm.warps || max_iterations < 0) return OF_ERR_ARG;
     if (!h->prepared) return OF_ERR_STATE;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     if (warps == 0) warps = h->prm.warps;
     if (max_iterations == 0) max_iterations = h->prm.max_iterations;
     return solve_levels(h, dUinit, warps, max_iterations, dU, st);
@@ -665,7 +665,7 @@ int of_tvl1_scale_flow(of_tvl1* h, int s, float* dUs, void* stream)
 {
     if (!h || s < 0 || s >= h->ns || !dUs) return OF_ERR_ARG;
     if (!h->solved) return OF_ERR_STATE;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     const size_t N = (size_t)h->nx[s] * h->ny[s];
     float* ub = h->ubase + 4 * h->off[s];
     OF_TRY(launch_copy_flow(ub, ub + 2 * N, dUs, 2 * N, h->ctl, s, st));
@@ -677,7 +677,7 @@ int of_tvl1_energy(of_tvl1* h, int s, const float* dUs, double* energy, void* st
 {
     if (!h || s < 0 || s >= h->ns || !dUs) return OF_ERR_ARG;
     if (!h->prepared) return OF_ERR_STATE;
-    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
     const size_t N = (size_t)h->nx[s] * h->ny[s];
     OF_TRY(launch_energy(h->I0p + h->off[s], h->I1p + h->off[s], dUs, dUs + N, h->nx[s], h->ny[s],
                          h->prm.lambda, h->energy, kEnergyBlocks, h->energy + 2 * kEnergyBlocks, st));
