@@ -34,6 +34,7 @@ namespace {
 
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ short to_bf16(float f)
@@ -49,11 +50,13 @@ __host__ __device__ __forceinline__ short to_bf16(float f)
 // (l % 16) of B for k = 8*(l/16) .. +7, and receives column (l % 16), rows
 // 4*(l/16) .. +3 of the 16x16 product (of_dbg_cnn_mfma checks this through
 // k_conv).
-__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ bf16x8 to_bf16x8(f32x8 v) { return __builtin_convertvector(v, bf16x8); }
 
 struct Pix {
     int b, y, x;
@@ -86,10 +89,10 @@ __device__ __forceinline__ void pix_next(Pix& p, int H, int W)
 // (k = t*CX + c), gathered from an NHWC tensor with CX channels; `sgn` = -1
 // mirrors the tap offsets (data gradient).
 template <int CX>
-__device__ __forceinline__ s16x8 gather8(const float* __restrict__ X, const Pix& p, bool valid,
-                                         int k, int K, int H, int W, int sgn)
+__device__ __forceinline__ bf16x8 gather8(const float* __restrict__ X, const Pix& p, bool valid,
+                                          int k, int K, int H, int W, int sgn)
 {
-    s16x8 r;
+    f32x8 r;
     if constexpr (CX % 8 == 0) {
         // the eight entries share a tap: two 16-byte loads
         const int t = k / CX, c = k - t * CX;
@@ -100,8 +103,8 @@ __device__ __forceinline__ s16x8 gather8(const float* __restrict__ X, const Pix&
             v0 = q[0];
             v1 = q[1];
         }
-        r[0] = to_bf16(v0.x); r[1] = to_bf16(v0.y); r[2] = to_bf16(v0.z); r[3] = to_bf16(v0.w);
-        r[4] = to_bf16(v1.x); r[5] = to_bf16(v1.y); r[6] = to_bf16(v1.z); r[7] = to_bf16(v1.w);
+        r[0] = v0.x; r[1] = v0.y; r[2] = v0.z; r[3] = v0.w;
+        r[4] = v1.x; r[5] = v1.y; r[6] = v1.z; r[7] = v1.w;
     } else {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -111,10 +114,10 @@ __device__ __forceinline__ s16x8 gather8(const float* __restrict__ X, const Pix&
             float v = 0.0f;
             if (valid && kk < K && yy >= 0 && yy < H && xx >= 0 && xx < W)
                 v = X[(((long long)p.b * H + yy) * W + xx) * CX + c];
-            r[j] = to_bf16(v);
+            r[j] = v;
         }
     }
-    return r;
+    return to_bf16x8(r);
 }
 
 // Implicit-GEMM 3x3 convolution.  A block is 4 waves; a wave owns 16
@@ -141,12 +144,14 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ X, int B
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < Kpad; k0 += 32) {
-        const s16x8 a = gather8<CX>(X, p, valid, k0 + 8 * kg, K, H, W, sgn);
+    constexpr int KP = (9 * CX + 31) / 32 * 32;   // == Kpad (checked by the launcher)
+#pragma unroll
+    for (int k0 = 0; k0 < KP; k0 += 32) {
+        const bf16x8 a = gather8<CX>(X, p, valid, k0 + 8 * kg, K, H, W, sgn);
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const s16x8 b =
-                *reinterpret_cast<const s16x8*>(Wp + (size_t)(t * 16 + row) * Kpad + k0 + 8 * kg);
+            const bf16x8 b = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const s16x8*>(Wp + (size_t)(t * 16 + row) * KP + k0 + 8 * kg));
             acc[t] = mfma32(a, b, acc[t]);
         }
     }
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY, int
     const int n = n0 + row;   // A row
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
     for (long long q0 = p_begin + wave * 32; q0 < p_end; q0 += 128) {
-        s16x8 a, b;
+        f32x8 a, b;
         const long long qf = q0 + 8 * kg;
         Pix p = pix_of(qf < M ? qf : 0, H, W);
 #pragma unroll
@@ -211,11 +216,11 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dY, int
                         vb = X[(((long long)p.b * H + y) * W + x) * CX + c];
                 }
             }
-            a[j] = to_bf16(va);
-            b[j] = to_bf16(vb);
+            a[j] = va;
+            b[j] = vb;
             pix_next(p, H, W);
         }
-        acc = mfma32(a, b, acc);
+        acc = mfma32(to_bf16x8(a), to_bf16x8(b), acc);
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) sh[wave][4 * kg + j][row] = acc[j];   // [n][k]
@@ -401,7 +406,7 @@ int launch_conv(const float* X, int B, int H, int W, int CX, const short* Wp, in
     const long long M = (long long)B * H * W;
     const unsigned blocks = (unsigned)((M + 63) / 64);
     const int NT = N <= 16 ? 1 : 2;
-    if (N > 32 || Kpad < 9 * CX || Kpad % 32) return OF_ERR_UNSUPPORTED;
+    if (N > 32 || Kpad != (9 * CX + 31) / 32 * 32) return OF_ERR_UNSUPPORTED;
 #define CONV_CASE(cx, nt)                                                                          \
     if (CX == cx && NT == nt) {                                                                    \
         hipLaunchKernelGGL((k_conv<cx, nt>), dim3(blocks), dim3(256), 0, st, X, B, H, W, Wp, Kpad, \

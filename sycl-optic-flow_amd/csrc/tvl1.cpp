@@ -441,7 +441,10 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
             const int row = mode * nshape + c;
             h->rules.assign(1, LaunchRule{0, kShapes[c][0], kShapes[c][1], mode});
             int st = solve(h, d0, d1, true, h->dflow, h->stream);   // warm-up of this variant
-            if (st == OF_ERR_UNSUPPORTED) {   // no cooperative launch on this device
+            // a persistent variant the device cannot run (no cooperative
+            // launch) or whose grid barrier ran past its bound is skipped;
+            // the kernels exit in both cases and the device stays usable
+            if (mode == 1 && st != OF_OK && hipDeviceSynchronize() == hipSuccess) {
                 if (table)
                     for (int s = 0; s < h->ns; s++) table[(size_t)row * h->ns + s] = -1.0;
                 continue;

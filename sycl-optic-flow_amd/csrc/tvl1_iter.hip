@@ -390,6 +390,12 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
                     ok = 0;
                     break;
                 }
+                // another workgroup gave up: stop waiting too
+                if ((spins & 1023u) == 0 &&
+                    __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
+                }
             }
         }
         if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
@@ -530,7 +536,9 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
             (void)hipGetLastError();
             return OF_ERR_HIP;
         }
-        resident = coop ? per_cu * cus : 0;
+        // at most 4 workgroups per CU: 1024 arrivals on one counter and
+        // pollers of one word per barrier are already plenty
+        resident = coop ? std::min(per_cu, 4) * cus : 0;
     }
     if (resident <= 0) return OF_ERR_UNSUPPORTED;
     unsigned gx, gy;

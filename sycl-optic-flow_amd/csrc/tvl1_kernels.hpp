@@ -23,10 +23,12 @@ struct TvCtl {
     unsigned wlaunches;         // the same, counted within the current warping
     unsigned long long maxkey[2];   // (|x| bits << 32) | (~index), per image
     unsigned long long minkey[2];   // (|x| bits << 32) | index, per image
-    // grid barrier of the persistent warping kernel
-    unsigned bar_count;
-    unsigned bar_gen;
-    int error;                      // set if a barrier wait ran past its bound
+    // grid barrier of the persistent warping kernel: the arrival counter and
+    // the generation word that waiting workgroups poll sit on cache lines of
+    // their own, away from each other and from the fields above
+    alignas(256) unsigned bar_count;
+    alignas(256) unsigned bar_gen;
+    alignas(256) int error;         // set if a barrier wait ran past its bound
 };
 
 struct IterArgs {

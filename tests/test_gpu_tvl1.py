@@ -193,10 +193,11 @@ def test_tvl1_launch_shapes_agree(K, R, persistent):
     assert np.array_equal(u, uo)
 
 
-@pytest.mark.parametrize("K,R", [(2, 8), (4, 4)])
+@pytest.mark.parametrize("K,R", [(2, 8), (4, 4), (1, 2), (2, 4)])
 def test_tvl1_persistent_everywhere_1080p(K, R):
     """The whole 1920x1080 solve with persistent warpings at every scale
-    (at 1920x1080 each workgroup loops over several tiles)."""
+    (at 1920x1080 each workgroup loops over several tiles; (1,2) and (2,4)
+    would fit 8 workgroups per CU, the grid is capped at 4)."""
     cfg = synth.CONFIGS["tvl1_1920x1080"]
     I0, I1, _, _ = synth.make_pair(cfg["width"], cfg["height"], cfg["cap"], 2024, bumps=True)
     q0, q1 = synth.quantize(I0), synth.quantize(I1)
