@@ -33,11 +33,27 @@ class OpticalFlowError(RuntimeError):
         self.status = status
 
 
+def _one_hip_runtime() -> None:
+    """PyTorch-ROCm ships its own HIP runtime (NEEDED as "libamdhip64.so",
+    soname "libamdhip64.so.7"); libamdflow.so needs "libamdhip64.so.7".  If
+    torch is loaded first, the dynamic loader binds libamdflow.so to torch's
+    runtime.  In the other order two HIP runtimes end up in the process and
+    the second one to initialise finds no device.  So when torch is
+    installed it is imported first (AMDFLOW_NO_TORCH=1 skips this)."""
+    import importlib.util
+    import sys
+    if "torch" in sys.modules or os.environ.get("AMDFLOW_NO_TORCH") == "1":
+        return
+    if importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401
+
+
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() "
             "(the HIP library is required, there is no CPU fallback)")
+    _one_hip_runtime()
     return C.CDLL(LIB_PATH)
 
 
