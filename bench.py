@@ -47,6 +47,8 @@ def parse():
                     help="pairs of the warm-start measurement (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for FETCH/WRITE_SIZE)")
+    ap.add_argument("--copy-too", action="store_true",
+                    help="run that copy first, then the benchmark (one counter pass covers both)")
     return ap.parse_args()
 
 
@@ -260,9 +262,10 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(pairs are independent: N GPUs would each solve their own share)", file=sys.stderr)
         sys.exit(2)
-    if args.calibrate_copy:
+    if args.calibrate_copy or args.copy_too:
         calibrate_copy()
-        return
+        if args.calibrate_copy:
+            return
 
     import numpy as np
     import torch
