@@ -172,7 +172,9 @@ int of_tvl1_profile_get(const of_tvl1 *h, of_tvl1_kernel_profile *out, int cap, 
  * Batched TV-L1 (no reference counterpart): many frame pairs of mixed sizes
  * solved concurrently by `streams` host workers, each with its own HIP
  * stream and one solver handle per frame size, taking pairs longest first.
- * Results are identical to solving each pair alone.
+ * Results are identical to solving each pair alone.  The workers' streams
+ * do not wait for other streams: the frames must be complete on the device
+ * when of_tvl1_batch_run_u8 is called.
  * ---------------------------------------------------------------------- */
 typedef struct of_tvl1_batch of_tvl1_batch;
 int of_tvl1_batch_create(const of_tvl1_params *params, int streams, of_tvl1_batch **out);
@@ -200,13 +202,24 @@ int of_tvl1_prepare_device_u8(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1
  * of_tvl1_scale_size), valid until the next prepare or run.  OF_ERR_STATE
  * before the first prepare. */
 int of_tvl1_scale_images(const of_tvl1 *h, int s, const float **dI0s, const float **dI1s);
-/* Phase 2 (tvl1.cpp:157-172): coarse-to-fine solve from `dUinit`, a device
- * planar [u1|u2] flow at the coarsest scale in that scale's pixel units, or
- * zero flow when NULL (as the reference).  At most `warps` warpings per
- * scale (<= the handle's) and `max_iterations` per warping; 0 selects the
- * handle's value.  Flow into dU (device, 2*width*height floats). */
-int of_tvl1_solve_prepared(of_tvl1 *h, const float *dUinit, int warps, int max_iterations,
-                           float *dU, void *stream);
+/* Phase 2 (tvl1.cpp:157-172): coarse-to-fine solve of scales first_scale
+ * .. 0 (first_scale = -1: the coarsest, as the reference) from `dUinit`, a
+ * device planar [u1|u2] flow at scale first_scale in that scale's pixel
+ * units, or zero flow when NULL (as the reference).  At most `warps`
+ * warpings per scale (<= the handle's) and `max_iterations` per warping; 0
+ * selects the handle's value.  Flow into dU (device, 2*width*height
+ * floats).  May be called again on the same prepared pair. */
+int of_tvl1_solve_prepared(of_tvl1 *h, const float *dUinit, int first_scale, int warps,
+                           int max_iterations, float *dU, void *stream);
+/* Carries a planar flow from scale from_scale to the finer scale to_scale
+ * the way the solve does between scales (zoom_in, zoom.cpp:61-85, times
+ * 1/zfactor per scale, tvl1.cpp:167-172). */
+int of_tvl1_lift_flow(of_tvl1 *h, int from_scale, const float *dUfrom, int to_scale, float *dUto,
+                      void *stream);
+/* Resamples a full-resolution planar flow to scale s: zoom_out
+ * (zoom.cpp:20-53) of each component s times, times zfactor^s (training
+ * labels of the warm start). */
+int of_tvl1_flow_to_scale(of_tvl1 *h, const float *dU, int s, float *dUs, void *stream);
 /* Copies the flow that scale s ended with in the last solve (planar, that
  * scale's size and units) into dUs. */
 int of_tvl1_scale_flow(of_tvl1 *h, int s, float *dUs, void *stream);
@@ -247,14 +260,17 @@ int of_cnn_gradients(of_cnn *h, const float *dX, const float *dT, int B, int H, 
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss. */
 int of_cnn_train_step(of_cnn *h, const float *dX, const float *dT, int B, int H, int W, float lr,
                       double *loss);
-/* Warm-started TV-L1 of one device u8 pair: predicts the coarsest flow,
- * keeps it if its energy (of_tvl1_energy) is not above zero flow's, and
- * solves with at most `warps` warpings and `max_iterations` per warping;
- * otherwise (*fell_back = 1) runs the plain solve with the handle's
- * parameters.  energies (optional): [0] predicted, [1] zero flow. */
+/* Warm-started TV-L1 of one device u8 pair: predicts the coarsest-scale
+ * flow, carries it to scale first_scale (-1: the coarsest) with
+ * of_tvl1_lift_flow, and keeps it if its energy there (of_tvl1_energy) is
+ * not above zero flow's: then scales first_scale .. 0 are solved with at
+ * most `warps` warpings and `max_iterations` per warping.  Otherwise
+ * (*fell_back = 1) the plain solve runs (zero flow at the coarsest scale,
+ * the handle's parameters).  energies (optional): [0] predicted, [1] zero
+ * flow, at first_scale. */
 int of_tvl1_warm_run_device_u8(of_tvl1 *tv, of_cnn *net, const uint8_t *dI0, const uint8_t *dI1,
-                               int warps, int max_iterations, float *dU, int *fell_back,
-                               double *energies, void *stream);
+                               int first_scale, int warps, int max_iterations, float *dU,
+                               int *fell_back, double *energies, void *stream);
 
 /* ------------------------------------------------------------------------
  * Synthetic frame pairs on the device (no reference counterpart; the

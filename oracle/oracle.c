@@ -533,12 +533,14 @@ void orc_tvl1_default_params(orc_tvl1_params *p)
  * (tvl1.cpp:160-165 converts u8 with a plain cast).  `u` receives planar
  * [u1 | u2] (2*W*H floats).  `iters` receives nscales*warps counts, finest
  * scale first.  Returns the number of scales used via *nscales_out. */
-/* The multiscale driver (tvl1.cpp:139-175).  `uinit` (planar [u1|u2] at the
- * coarsest scale) replaces the zero initial flow of tvl1.cpp:157-158 when
- * not NULL: the warm-start extension, which has no reference counterpart. */
+/* The multiscale driver (tvl1.cpp:139-175).  The warm-start extension,
+ * which has no reference counterpart: the solve starts at scale `first`
+ * (-1: the coarsest, as the reference) from `uinit` (planar [u1|u2] at that
+ * scale) instead of the zero flow of tvl1.cpp:157-158 when not NULL; the
+ * coarser scales are not solved. */
 static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
-                    const orc_tvl1_params *prm, const float *uinit, float *u, int *iters,
-                    int *nscales_out)
+                    const orc_tvl1_params *prm, const float *uinit, int first, float *u,
+                    int *iters, int *nscales_out)
 {
     const int ns = orc_tvl1_nscales(width, height, prm->nscales, prm->zfactor);
     if (nscales_out) *nscales_out = ns;
@@ -566,14 +568,17 @@ static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
         st = orc_zoom_out(I0s[s - 1], I0s[s], nx[s - 1], ny[s - 1], prm->zfactor, scratch);
         if (!st) st = orc_zoom_out(I1s[s - 1], I1s[s], nx[s - 1], ny[s - 1], prm->zfactor, scratch);
     }
+    if (first < 0 || first >= ns) first = ns - 1;
+    if (iters)
+        for (int i = 0; i < ns * prm->warps; i++) iters[i] = 0;
     if (!st) {
-        const size_t nc = (size_t)nx[ns - 1] * ny[ns - 1];
+        const size_t nc = (size_t)nx[first] * ny[first];
         for (size_t i = 0; i < nc; i++) {
-            u1s[ns - 1][i] = uinit ? uinit[i] : 0.0f;
-            u2s[ns - 1][i] = uinit ? uinit[nc + i] : 0.0f;
+            u1s[first][i] = uinit ? uinit[i] : 0.0f;
+            u2s[first][i] = uinit ? uinit[nc + i] : 0.0f;
         }
         const float invZ = 1 / prm->zfactor;
-        for (int s = ns - 1; s > 0 && !st; s--) {
+        for (int s = first; s > 0 && !st; s--) {
             st = orc_dual_tvl1(I0s[s], I1s[s], u1s[s], u2s[s], nx[s], ny[s], prm,
                                iters ? iters + (size_t)s * prm->warps : NULL);
             orc_zoom_in(u1s[s], u1s[s - 1], nx[s], ny[s], nx[s - 1], ny[s - 1]);
@@ -598,13 +603,13 @@ static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
 int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height,
                      const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
 {
-    return tvl1_run(I0in, I1in, width, height, prm, NULL, u, iters, nscales_out);
+    return tvl1_run(I0in, I1in, width, height, prm, NULL, -1, u, iters, nscales_out);
 }
 
 /* tvl1.cpp:160-165 then the multiscale driver, from `uinit` (or zero). */
 int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
-                         const orc_tvl1_params *prm, const float *uinit, float *u, int *iters,
-                         int *nscales_out)
+                         const orc_tvl1_params *prm, const float *uinit, int first, float *u,
+                         int *iters, int *nscales_out)
 {
     const size_t size = (size_t)width * height;
     float *f0 = (float *)malloc(sizeof(float) * size);
@@ -614,7 +619,7 @@ int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int he
         f0[i] = (float)I0[i];
         f1[i] = (float)I1[i];
     }
-    const int st = tvl1_run(f0, f1, width, height, prm, uinit, u, iters, nscales_out);
+    const int st = tvl1_run(f0, f1, width, height, prm, uinit, first, u, iters, nscales_out);
     free(f0);
     free(f1);
     return st;
@@ -623,7 +628,7 @@ int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int he
 int orc_tvl1_run_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
                     const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
 {
-    return orc_tvl1_run_init_u8(I0, I1, width, height, prm, NULL, u, iters, nscales_out);
+    return orc_tvl1_run_init_u8(I0, I1, width, height, prm, NULL, -1, u, iters, nscales_out);
 }
 
 /* TV-L1 energy of a flow (no reference counterpart; the warm start's

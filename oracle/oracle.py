@@ -70,8 +70,8 @@ _sig("orc_tvl1_iteration", C.c_double, [_vp] * 9 + [C.c_int, C.c_int, C.c_float,
 _sig("orc_dual_tvl1", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _ip])
 _sig("orc_tvl1_run_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _ip, _ip])
 _sig("orc_tvl1_run_f32", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _ip, _ip])
-_sig("orc_tvl1_run_init_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _vp, _ip,
-                                       _ip])
+_sig("orc_tvl1_run_init_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, C.c_int,
+                                       _vp, _ip, _ip])
 _sig("orc_tvl1_energy", C.c_double, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)])
 _sig("orc_bilinear_at", C.c_float, [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float])
@@ -262,10 +262,11 @@ def tvl1(I0, I1, params: Params | None = None):
     return u, iters.reshape(ns, p.warps)
 
 
-def tvl1_init(I0, I1, uinit, params: Params | None = None):
-    """Multiscale TV-L1 of a uint8 pair from the initial coarsest-scale flow
-    `uinit` (planar [u1|u2] at that scale; None = zero, the reference's
-    start).  Returns (u, iteration counts [nscales, warps])."""
+def tvl1_init(I0, I1, uinit, params: Params | None = None, first: int = -1):
+    """Multiscale TV-L1 of a uint8 pair solving scales first .. 0 (-1: all,
+    as the reference) from the initial flow `uinit` at scale `first` (planar
+    [u1|u2]; None = zero, the reference's start).  Returns (u, iteration
+    counts [nscales, warps], zero for unsolved scales)."""
     p = params or default_params()
     I0 = np.ascontiguousarray(I0, dtype=np.uint8)
     h, w = I0.shape
@@ -276,7 +277,7 @@ def tvl1_init(I0, I1, uinit, params: Params | None = None):
     b = np.ascontiguousarray(I1, dtype=np.uint8)
     ui = None if uinit is None else _f32(uinit)
     st = _lib.orc_tvl1_run_init_u8(_p(I0), _p(b), w, h, C.byref(p), None if ui is None else _p(ui),
-                                   _p(u), iters.ctypes.data_as(_ip), C.byref(nso))
+                                   int(first), _p(u), iters.ctypes.data_as(_ip), C.byref(nso))
     if st:
         raise ValueError(f"tvl1_init: status {st}")
     return u, iters.reshape(ns, p.warps)

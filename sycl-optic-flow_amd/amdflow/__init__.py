@@ -114,7 +114,9 @@ SIGNATURES = {
     "of_tvl1_profile_get": (C.c_int, [_vp, C.POINTER(TVL1KernelProfile), C.c_int, _ip]),
     "of_tvl1_prepare_device_u8": (C.c_int, [_vp, _vp, _vp, _vp]),
     "of_tvl1_scale_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
-    "of_tvl1_solve_prepared": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp]),
+    "of_tvl1_solve_prepared": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "of_tvl1_lift_flow": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, _vp]),
+    "of_tvl1_flow_to_scale": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
     "of_tvl1_scale_flow": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "of_tvl1_energy": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(C.c_double), _vp]),
     "of_hs_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
@@ -304,10 +306,19 @@ class TV_L1:
         _check(_lib.of_tvl1_scale_images(self._h, int(s), C.byref(a), C.byref(b)), "scale_images")
         return a.value, b.value
 
-    def solve_prepared(self, d_U: int, d_Uinit: int = 0, warps: int = 0, max_iterations: int = 0,
-                       stream: int = 0) -> None:
-        _check(_lib.of_tvl1_solve_prepared(self._h, d_Uinit or None, int(warps), int(max_iterations),
-                                           d_U, stream or None), "solve_prepared")
+    def solve_prepared(self, d_U: int, d_Uinit: int = 0, first_scale: int = -1, warps: int = 0,
+                       max_iterations: int = 0, stream: int = 0) -> None:
+        """Solves scales first_scale .. 0 (-1: all) of the prepared pair from
+        the device flow d_Uinit at first_scale (0: zero flow)."""
+        _check(_lib.of_tvl1_solve_prepared(self._h, d_Uinit or None, int(first_scale), int(warps),
+                                           int(max_iterations), d_U, stream or None), "solve_prepared")
+
+    def lift_flow(self, from_scale: int, d_from: int, to_scale: int, d_to: int, stream: int = 0) -> None:
+        _check(_lib.of_tvl1_lift_flow(self._h, int(from_scale), d_from, int(to_scale), d_to,
+                                      stream or None), "lift_flow")
+
+    def flow_to_scale(self, d_U: int, s: int, d_Us: int, stream: int = 0) -> None:
+        _check(_lib.of_tvl1_flow_to_scale(self._h, d_U, int(s), d_Us, stream or None), "flow_to_scale")
 
     def scale_flow(self, s: int, d_Us: int, stream: int = 0) -> None:
         _check(_lib.of_tvl1_scale_flow(self._h, int(s), d_Us, stream or None), "scale_flow")

@@ -28,8 +28,8 @@ _SIGS = {
     "of_cnn_predict": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
     "of_cnn_gradients": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _dp]),
     "of_cnn_train_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _dp]),
-    "of_tvl1_warm_run_device_u8": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _ip, _dp,
-                                             _vp]),
+    "of_tvl1_warm_run_device_u8": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _ip,
+                                             _dp, _vp]),
     "of_dbg_cnn_mfma": (C.c_int, [_vp, _vp, _vp]),
 }
 for _n, (_r, _a) in _SIGS.items():
@@ -38,6 +38,7 @@ for _n, (_r, _a) in _SIGS.items():
     _f.argtypes = _a
 
 HIDDEN = 5          # 6 conv layers of 32 channels, then the 2-channel head
+FIRST_SCALE = 1     # first solved scale of a warm-started solve (960x540 at 1920x1080)
 CHANNELS = 32
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "weights",
                        "warmstart_c32_h5.npy")
@@ -121,14 +122,16 @@ class WarmStartNet:
                                       C.byref(loss)), "cnn_train_step")
         return loss.value
 
-    def warm_run(self, tv, d_I0: int, d_I1: int, d_U: int, warps: int = 2,
-                 max_iterations: int = 100, stream: int = 0):
-        """Warm-started TV-L1 of a device u8 pair on solver `tv`.  Returns
-        (fell_back, energy of the prediction, energy of zero flow)."""
+    def warm_run(self, tv, d_I0: int, d_I1: int, d_U: int, first_scale: int = FIRST_SCALE,
+                 warps: int = 2, max_iterations: int = 100, stream: int = 0):
+        """Warm-started TV-L1 of a device u8 pair on solver `tv`: the
+        predicted coarsest flow, carried to `first_scale`, seeds the solve of
+        scales first_scale .. 0.  Returns (fell_back, energy of the
+        prediction, energy of zero flow) at first_scale."""
         fb = C.c_int()
         e = (C.c_double * 2)()
-        _check(_lib.of_tvl1_warm_run_device_u8(tv.handle, self._h, d_I0, d_I1, int(warps),
-                                               int(max_iterations), d_U, C.byref(fb), e,
+        _check(_lib.of_tvl1_warm_run_device_u8(tv.handle, self._h, d_I0, d_I1, int(first_scale),
+                                               int(warps), int(max_iterations), d_U, C.byref(fb), e,
                                                stream or None), "warm_run")
         return bool(fb.value), e[0], e[1]
 

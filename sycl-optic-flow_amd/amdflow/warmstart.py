@@ -3,15 +3,22 @@
 Data: seeded synthetic 1920x1080 pairs generated on the GPU (of_synth_pair,
 sinusoid + bump motion up to 20 px, noise sigma 1), each solved by the plain
 TV-L1 solver (5 scales, 5 warpings, at most 300 iterations, zero initial
-flow).  The network input is the coarsest scale's (I0, I1, Ix, Iy, It), the
-label is the flow the plain solver ends that scale with.  Training: Adam,
-learning rate 1e-3, mean-squared flow error, on the GPU through the
-library's own forward / backward kernels.
+flow).  The network input is the coarsest scale's (I0, I1, Ix, Iy, It); the
+label is the plain solver's final flow resampled to the coarsest scale
+(of_tvl1_flow_to_scale).  Training: Adam, learning rate 1e-3, mean-squared
+flow error, on the GPU through the library's own forward / backward kernels.
 
-Evaluation: the warm-started solve (at most `warps` warpings and
-`max_iterations` iterations per warping, energy-checked fallback to the
-plain solve) and the same limited solve from zero flow are both compared
-with the converged plain solve by mean end-point error (EPE).
+Use: the prediction, carried to scale `first_scale` the way the solver
+carries flow between scales, seeds a solve of scales first_scale .. 0 with
+at most `warps` warpings and `max_iterations` iterations per warping; the
+coarser scales are not solved.  (Seeding the coarsest scale instead gains
+nothing: the coarse scales of TV-L1 pull any initial flow towards their own
+solution, which is close to zero at 120x68.)  Evaluation compares, by mean
+end-point error (EPE), against the plain solve and the true motion: the
+warm-started solve, the same limited solve from zero flow (the method
+without the network), and the same limited solve seeded with the plain
+flow itself resampled to the coarsest scale (what a perfect network would
+give).
 
     PYTHONPATH=sycl-optic-flow_amd python -m amdflow.warmstart --pairs 20000 \
         --epochs 8 --out sycl-optic-flow_amd/weights/warmstart_c32_h5.npy
@@ -25,7 +32,7 @@ import time
 import numpy as np
 
 from . import TV_L1, tuning
-from .cnn import WarmStartNet
+from .cnn import FIRST_SCALE, WarmStartNet
 from .synth import DevicePairs
 
 WIDTH, HEIGHT, CAP = 1920, 1080, 20.0
@@ -61,7 +68,7 @@ def make_dataset(npairs: int, seed0: int = TRAIN_SEED0, width: int = WIDTH, heig
         tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
         a, b = tv.scale_images(ns - 1)
         WarmStartNet.features(a, b, hc, wc, X[i].data_ptr(), sp)
-        tv.scale_flow(ns - 1, T[i].data_ptr(), sp)
+        tv.flow_to_scale(dU.data_ptr(), ns - 1, T[i].data_ptr(), sp)
         if log and (i + 1) % every == 0:
             log(f"dataset: {i + 1}/{npairs} pairs, {time.perf_counter() - t0:.1f} s")
     stream.synchronize()
@@ -100,11 +107,12 @@ def epe(a, b, n: int) -> float:
     return float(d.square().sum(0).sqrt().mean())
 
 
-def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0, warps: int = 2,
-             max_iterations: int = 100, width: int = WIDTH, height: int = HEIGHT, cap: float = CAP,
-             reps: int = 2):
-    """EPE against the converged plain solve and pairs/s of the plain,
-    warm-started and zero-initialised limited solves over `npairs` pairs."""
+def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0,
+             first_scale: int = FIRST_SCALE, warps: int = 2, max_iterations: int = 100,
+             width: int = WIDTH, height: int = HEIGHT, cap: float = CAP, reps: int = 2):
+    """EPE against the converged plain solve and the true motion, and pairs/s,
+    of the plain solve, the warm-started solve and the same limited solve
+    from zero flow, over `npairs` held-out pairs."""
     import torch
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
@@ -117,9 +125,16 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0, warps
         pairs.append((a.clone(), b.clone(), f.clone()))
     torch.cuda.synchronize()
     tv = _solver(width, height)
+    ns = tv.nscales
+    fs = ns - 1 if first_scale < 0 else first_scale
+    cw, ch = tv.scale_size(ns - 1)
+    fw, fh = tv.scale_size(fs)
+    kinds = ("plain", "warm", "zero", "perfect")
     outs = {k: [torch.empty(2 * n, dtype=torch.float32, device="cuda") for _ in range(npairs)]
-            for k in ("plain", "warm", "zero")}
-    fell, e_pred, e_zero, its = [], [], [], {"plain": 0, "warm": 0, "zero": 0}
+            for k in kinds}
+    dC = torch.empty(2 * cw * ch, dtype=torch.float32, device="cuda")
+    dL = torch.empty(2 * fw * fh, dtype=torch.float32, device="cuda")
+    fell, e_pred, e_zero, its = [], [], [], {k: 0 for k in kinds}
 
     def run(kind, i):
         d0, d1, _ = pairs[i]
@@ -127,21 +142,28 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0, warps
         if kind == "plain":
             tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU, sp)
         elif kind == "warm":
-            fb, ep, ez = net.warm_run(tv, d0.data_ptr(), d1.data_ptr(), dU, warps, max_iterations, sp)
-            return fb, ep, ez
-        else:
+            return net.warm_run(tv, d0.data_ptr(), d1.data_ptr(), dU, first_scale, warps,
+                                max_iterations, sp)
+        elif kind == "zero":
             tv.prepare_device_u8(d0.data_ptr(), d1.data_ptr(), sp)
-            tv.solve_prepared(dU, 0, warps, max_iterations, sp)
+            tv.solve_prepared(dU, 0, fs, warps, max_iterations, sp)
+        else:   # the plain flow itself, resampled to the coarsest scale, as the seed
+            tv.flow_to_scale(outs["plain"][i].data_ptr(), ns - 1, dC.data_ptr(), sp)
+            tv.prepare_device_u8(d0.data_ptr(), d1.data_ptr(), sp)
+            tv.lift_flow(ns - 1, dC.data_ptr(), fs, dL.data_ptr(), sp)
+            tv.solve_prepared(dU, dL.data_ptr(), fs, warps, max_iterations, sp)
         return None
 
     timing = {}
-    for kind in ("plain", "warm", "zero"):
+    for kind in kinds:
         for i in range(npairs):                  # warm-up and results
             r = run(kind, i)
             its[kind] += int(tv.iterations().sum())
             if kind == "warm":
                 fell.append(r[0]); e_pred.append(r[1]); e_zero.append(r[2])
         stream.synchronize()
+        if kind == "perfect":
+            continue   # not a method: it needs the plain solve first
         t0 = time.perf_counter()
         for _ in range(reps):
             for i in range(npairs):
@@ -149,17 +171,20 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0, warps
         stream.synchronize()
         timing[kind] = reps * npairs / (time.perf_counter() - t0)
     res = {
-        "pairs": npairs, "warps": warps, "max_iterations": max_iterations,
+        "pairs": npairs, "first_scale": fs, "first_scale_size": [fw, fh], "warps": warps,
+        "max_iterations": max_iterations,
         "pairs_per_s": {k: round(v, 2) for k, v in timing.items()},
         "speedup_warm_vs_plain": round(timing["warm"] / timing["plain"], 3),
         "mean_epe_vs_plain": {
             "warm": float(np.mean([epe(outs["warm"][i], outs["plain"][i], n) for i in range(npairs)])),
             "zero_init_same_limits": float(np.mean([epe(outs["zero"][i], outs["plain"][i], n)
                                                     for i in range(npairs)])),
+            "perfect_seed_same_limits": float(np.mean([epe(outs["perfect"][i], outs["plain"][i], n)
+                                                       for i in range(npairs)])),
         },
         "mean_epe_vs_ground_truth": {
             k: float(np.mean([epe(outs[k][i], pairs[i][2], n) for i in range(npairs)]))
-            for k in ("plain", "warm", "zero")},
+            for k in kinds},
         "iterations_per_pair": {k: its[k] / npairs for k in its},
         "fallbacks": int(sum(fell)),
         "mean_energy_ratio_pred_vs_zero": float(np.mean(np.asarray(e_pred) / np.asarray(e_zero))),

@@ -26,7 +26,8 @@ struct of_tvl1_batch {
     of_tvl1_params prm{};
     int streams = 1;
     int device = 0;
-    // per worker: frame size -> handle
+    // per worker: its stream and its handles (frame size -> handle)
+    std::vector<hipStream_t> wstreams;
     std::vector<std::map<std::pair<int, int>, of_tvl1*>> handles;
     // launch rules applied to every handle: (min_pixels, K, R)
     std::vector<std::tuple<long long, int, int>> rules;
@@ -81,9 +82,13 @@ int of_tvl1_batch_create(const of_tvl1_params* params, int streams, of_tvl1_batc
     else of_tvl1_default_params(&b->prm);
     b->streams = streams;
     b->handles.resize(streams);
-    if (hipGetDevice(&b->device) != hipSuccess) {
+    b->wstreams.assign(streams, nullptr);
+    bool ok = hipGetDevice(&b->device) == hipSuccess;
+    for (int w = 0; w < streams && ok; w++)
+        ok = hipStreamCreateWithFlags(&b->wstreams[w], hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
         (void)hipGetLastError();
-        delete b;
+        of_tvl1_batch_destroy(b);
         return OF_ERR_HIP;
     }
     *out = b;
@@ -95,6 +100,8 @@ void of_tvl1_batch_destroy(of_tvl1_batch* b)
     if (!b) return;
     for (auto& m : b->handles)
         for (auto& kv : m) of_tvl1_destroy(kv.second);
+    for (hipStream_t s : b->wstreams)
+        if (s) (void)hipStreamDestroy(s);
     delete b;
 }
 
@@ -139,7 +146,7 @@ int of_tvl1_batch_run_u8(of_tvl1_batch* b, int npairs, const int* widths, const 
             const int i = order[k];
             of_tvl1* h = nullptr;
             int st = worker_handle(b, w, widths[i], heights[i], &h);
-            if (!st) st = of_tvl1_run_device_u8(h, dI0[i], dI1[i], dU[i], nullptr);
+            if (!st) st = of_tvl1_run_device_u8(h, dI0[i], dI1[i], dU[i], b->wstreams[w]);
             if (!st && iterations_total) {
                 std::vector<int> c((size_t)of_tvl1_nscales(h) * b->prm.warps);
                 of_tvl1_iterations(h, c.data(), (int)c.size());

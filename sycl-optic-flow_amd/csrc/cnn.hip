@@ -472,8 +472,12 @@ struct of_cnn {
     double* loss = nullptr;
     // single-image inference workspace
     long long cap_in = 0;
-    float* xin = nullptr;      // 5-channel features (also a zero field for the energy test)
+    float* xin = nullptr;      // 5-channel features
     float* pred = nullptr;     // planar prediction
+    // warm-start workspace: the prediction carried to the first solved scale, zero flow
+    long long cap_lift = 0;
+    float* lift = nullptr;
+    float* zero = nullptr;
 };
 
 namespace {
@@ -611,6 +615,8 @@ void release(of_cnn* h)
     dev_free(h->loss);
     dev_free(h->xin);
     dev_free(h->pred);
+    dev_free(h->lift);
+    dev_free(h->zero);
     delete h;
 }
 
@@ -780,39 +786,53 @@ int of_cnn_train_step(of_cnn* h, const float* dX, const float* dT, int B, int H,
 }
 
 // Warm-started solve of one pair: prepare the pyramids, predict the
-// coarsest-scale flow, keep it only if its TV-L1 energy at that scale is
-// not above that of zero flow, then solve with at most `warps` warpings and
-// `max_iterations` iterations per warping (0: the handle's parameters).
-// When zero flow wins, the solve is the plain one: zero initial flow and the
-// handle's own parameters.
+// coarsest-scale flow, carry it to `first_scale`, keep it only if its TV-L1
+// energy there is not above that of zero flow, then solve scales
+// first_scale .. 0 with at most `warps` warpings and `max_iterations`
+// iterations per warping (0: the handle's parameters).  When zero flow wins,
+// the solve is the plain one: zero initial flow at the coarsest scale and
+// the handle's own parameters.
 int of_tvl1_warm_run_device_u8(of_tvl1* tv, of_cnn* net, const uint8_t* dI0, const uint8_t* dI1,
-                               int warps, int max_iterations, float* dU, int* fell_back,
-                               double* energies, void* stream)
+                               int first_scale, int warps, int max_iterations, float* dU,
+                               int* fell_back, double* energies, void* stream)
 {
     if (!tv || !net || !dI0 || !dI1 || !dU || warps < 0 || max_iterations < 0) return OF_ERR_ARG;
     const int ns = of_tvl1_nscales(tv);
-    int nx = 0, ny = 0;
+    if (first_scale < -1 || first_scale >= ns) return OF_ERR_ARG;
+    if (first_scale < 0) first_scale = ns - 1;
+    int nx = 0, ny = 0, fx = 0, fy = 0;
     OF_TRY(of_tvl1_scale_size(tv, ns - 1, &nx, &ny));
-    const long long n = (long long)nx * ny;
+    OF_TRY(of_tvl1_scale_size(tv, first_scale, &fx, &fy));
+    const long long n = (long long)nx * ny, nf = (long long)fx * fy;
     OF_TRY(ensure_in(net, n));
+    if (2 * nf > net->cap_lift) {
+        OF_HIP(hipDeviceSynchronize());
+        dev_free(net->lift);
+        dev_free(net->zero);
+        net->cap_lift = 0;
+        OF_TRY(dev_alloc(&net->lift, 2 * (size_t)nf));
+        OF_TRY(dev_alloc(&net->zero, 2 * (size_t)nf));
+        net->cap_lift = 2 * nf;
+    }
+    hipStream_t st = (hipStream_t)stream;
     OF_TRY(of_tvl1_prepare_device_u8(tv, dI0, dI1, stream));
     const float* I0c = nullptr;
     const float* I1c = nullptr;
     OF_TRY(of_tvl1_scale_images(tv, ns - 1, &I0c, &I1c));
     OF_TRY(of_cnn_predict(net, I0c, I1c, ny, nx, net->pred, stream));
+    OF_TRY(of_tvl1_lift_flow(tv, ns - 1, net->pred, first_scale, net->lift, stream));
+    OF_HIP(hipMemsetAsync(net->zero, 0, sizeof(float) * 2 * (size_t)nf, st));
     double ep[3], ez[3];
-    OF_TRY(of_tvl1_energy(tv, ns - 1, net->pred, ep, stream));
-    // zero flow, in the (no longer needed) feature buffer
-    OF_HIP(hipMemsetAsync(net->xin, 0, sizeof(float) * 2 * n, (hipStream_t)stream));
-    OF_TRY(of_tvl1_energy(tv, ns - 1, net->xin, ez, stream));
+    OF_TRY(of_tvl1_energy(tv, first_scale, net->lift, ep, stream));
+    OF_TRY(of_tvl1_energy(tv, first_scale, net->zero, ez, stream));
     if (energies) {
         energies[0] = ep[0];
         energies[1] = ez[0];
     }
     const bool keep = ep[0] <= ez[0];
     if (fell_back) *fell_back = keep ? 0 : 1;
-    if (keep) return of_tvl1_solve_prepared(tv, net->pred, warps, max_iterations, dU, stream);
-    return of_tvl1_solve_prepared(tv, nullptr, 0, 0, dU, stream);
+    if (keep) return of_tvl1_solve_prepared(tv, net->lift, first_scale, warps, max_iterations, dU, stream);
+    return of_tvl1_solve_prepared(tv, nullptr, -1, 0, 0, dU, stream);
 }
 
 // ---- test entry point: MFMA operand layout through k_conv ----

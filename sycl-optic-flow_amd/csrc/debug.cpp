@@ -160,11 +160,14 @@ int of_dbg_warp_setup(const float* I0, const float* I1, const float* I1x, const 
     float* wy = B.alloc<float>(n);
     float* rc = B.alloc<float>(n);
     TvCtl* ctl = B.alloc<TvCtl>(1);
+    float* src = B.alloc<float>(4 * n);
     OF_NN(d0); OF_NN(d1); OF_NN(dx); OF_NN(dy); OF_NN(ub); OF_NN(wx); OF_NN(wy); OF_NN(rc); OF_NN(ctl);
+    OF_NN(src);
     OF_HIP(hipMemcpy(ub, u1, n * sizeof(float), hipMemcpyHostToDevice));
     OF_HIP(hipMemcpy(ub + n, u2, n * sizeof(float), hipMemcpyHostToDevice));
     OF_TRY(launch_ctl_init(ctl, nullptr));
-    OF_TRY(launch_warp_setup(d0, d1, dx, dy, ub, ub, wx, wy, rc, w, h, ctl, 0, nullptr));
+    OF_TRY(launch_pack3(d1, dx, dy, src, n, nullptr));
+    OF_TRY(launch_warp_setup(d0, src, ub, ub, wx, wy, rc, w, h, ctl, 0, nullptr));
     OF_HIP(hipMemcpy(I1wx, wx, n * sizeof(float), hipMemcpyDeviceToHost));
     OF_HIP(hipMemcpy(I1wy, wy, n * sizeof(float), hipMemcpyDeviceToHost));
     OF_HIP(hipMemcpy(rho_c, rc, n * sizeof(float), hipMemcpyDeviceToHost));

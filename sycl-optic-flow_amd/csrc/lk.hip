@@ -246,6 +246,87 @@ __global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ row
     }
 }
 
+// One whole iteration on a 64 x 16 output tile: the warp and the two
+// products on the tile plus a WC-pixel halo (LDS), their row sums, then the
+// column sums and the 2x2 update.  Same expressions and summation order as
+// k_lk_it_products / k_lk_boxsum / k_lk_update (products outside the image
+// are never added: the windows are clipped to the image); reads (u, v),
+// writes the updated flow to (uo, vo).
+template <int WC>
+__global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__ P0,
+                                                       const float* __restrict__ P1,
+                                                       const float* __restrict__ Ix,
+                                                       const float* __restrict__ Iy,
+                                                       const float* __restrict__ G,
+                                                       const float* __restrict__ u,
+                                                       const float* __restrict__ v, int w, int h,
+                                                       float det_eps, float* __restrict__ uo,
+                                                       float* __restrict__ vo)
+{
+    constexpr int TW = 64, TH = 16, EW = TW + 2 * WC, EH = TH + 2 * WC;
+    __shared__ float pr[2][EH][EW];
+    __shared__ float rs[2][EH][TW];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const size_t n = (size_t)w * h;
+    for (int e = threadIdx.x; e < EW * EH; e += 256) {
+        const int ey = e / EW, ex = e - ey * EW;
+        const int gx = x0 - WC + ex, gy = y0 - WC + ey;
+        float a = 0.0f, b = 0.0f;
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+            const size_t p = (size_t)gy * w + gx;
+            const float wv = bilinear_at(P1, w, h, (float)gx + u[p], (float)gy + v[p]);
+            const float itv = wv - P0[p];
+            a = Ix[p] * itv;
+            b = Iy[p] * itv;
+        }
+        pr[0][ey][ex] = a;
+        pr[1][ey][ex] = b;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TW * EH; e += 256) {
+        const int ey = e / TW, ox = e - ey * TW;
+        const int gx = x0 + ox;
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int k = -WC; k <= WC; k++) {
+            const int jj = gx + k;
+            if (jj >= 0 && jj < w) {
+                s0 += pr[0][ey][ox + WC + k];
+                s1 += pr[1][ey][ox + WC + k];
+            }
+        }
+        rs[0][ey][ox] = s0;
+        rs[1][ey][ox] = s1;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TW * TH; e += 256) {
+        const int oy = e / TW, ox = e - oy * TW;
+        const int gx = x0 + ox, gy = y0 + oy;
+        if (gx >= w || gy >= h) continue;
+        float bx = 0.0f, by = 0.0f;
+#pragma unroll
+        for (int k = -WC; k <= WC; k++) {
+            const int ii = gy + k;
+            if (ii >= 0 && ii < h) {
+                bx += rs[0][oy + WC + k][ox];
+                by += rs[1][oy + WC + k][ox];
+            }
+        }
+        const size_t p = (size_t)gy * w + gx;
+        const float gxx = G[p], gxy = G[n + p], gyy = G[2 * n + p];
+        const float det = gxx * gyy - gxy * gxy;
+        float nu = u[p], nv = v[p];
+        if (det > det_eps) {
+            const float du = (gxy * by - gyy * bx) / det;
+            const float dv = (gxy * bx - gxx * by) / det;
+            nu = nu + du;
+            nv = nv + dv;
+        }
+        uo[p] = nu;
+        vo[p] = nv;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_lk_zero2(float* __restrict__ a, float* __restrict__ b,
                                                   size_t n)
 {
@@ -260,7 +341,48 @@ unsigned g1(size_t n) { const unsigned g = cdiv(n, 256); return g > 4096 ? 4096 
 
 }  // namespace
 
-// Pyramidal LK from device frames.  Workspace sized for the finest level.
+// Device workspace of the pyramidal LK, one per host thread, grown on
+// demand and kept for the thread's lifetime (the entry points take no
+// handle).
+struct LkWorkspace {
+    size_t cap_total = 0, cap_n0 = 0;
+    float *P0 = nullptr, *P1 = nullptr, *A0 = nullptr, *A1 = nullptr, *Ix = nullptr, *Iy = nullptr;
+    float *PROD = nullptr, *ROWS = nullptr, *GT = nullptr, *u = nullptr, *v = nullptr, *nu = nullptr,
+          *nv = nullptr;
+    void release()
+    {
+        for (float** p : {&P0, &P1, &A0, &A1, &Ix, &Iy, &PROD, &ROWS, &GT, &u, &v, &nu, &nv}) dev_free(*p);
+        cap_total = cap_n0 = 0;
+    }
+    int ensure(size_t total, size_t n0)
+    {
+        if (total <= cap_total && n0 <= cap_n0) return OF_OK;
+        if (hipDeviceSynchronize() != hipSuccess) return OF_ERR_HIP;
+        release();
+        int st = dev_alloc(&P0, total);
+        if (!st) st = dev_alloc(&P1, total);
+        for (float** p : {&A0, &A1, &Ix, &Iy, &u, &v, &nu, &nv})
+            if (!st) st = dev_alloc(p, n0);
+        for (float** p : {&PROD, &ROWS, &GT})
+            if (!st) st = dev_alloc(p, 3 * n0);
+        if (st) {
+            release();
+            return st;
+        }
+        cap_total = total;
+        cap_n0 = n0;
+        return OF_OK;
+    }
+    ~LkWorkspace() { release(); }
+};
+
+LkWorkspace& lk_workspace()
+{
+    thread_local LkWorkspace ws;
+    return ws;
+}
+
+// Pyramidal LK from device frames.
 int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, float zf,
                  float sigma, int win, int iters, float det_eps, float* du, float* dv,
                  hipStream_t st)
@@ -282,29 +404,21 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
         total += (size_t)nx[l] * ny[l];
     }
     const size_t n0 = (size_t)W * H;
-    std::vector<float*> bufs;
-    auto alloc = [&](size_t n) -> float* {
-        float* p = nullptr;
-        if (dev_alloc(&p, n) != OF_OK) return nullptr;
-        bufs.push_back(p);
-        return p;
-    };
-    float* P0 = alloc(total);
-    float* P1 = alloc(total);
-    float* A0 = alloc(n0);
-    float* A1 = alloc(n0);
-    float* Ix = alloc(n0);
-    float* Iy = alloc(n0);
-    float* PROD = alloc(3 * n0);   // per-pixel products (3 or 2 planes of the level size)
-    float* ROWS = alloc(3 * n0);   // their row sums
-    float* GT = alloc(3 * n0);     // structure tensor Gxx, Gxy, Gyy of the level
-    float* u = alloc(n0);
-    float* v = alloc(n0);
-    float* nu = alloc(n0);
-    float* nv = alloc(n0);
-    int status = OF_OK;
-    if (!P0 || !P1 || !A0 || !A1 || !Ix || !Iy || !PROD || !ROWS || !GT || !u || !v || !nu || !nv)
-        status = OF_ERR_ALLOC;
+    LkWorkspace& ws = lk_workspace();
+    int status = ws.ensure(total, n0);
+    float* P0 = ws.P0;
+    float* P1 = ws.P1;
+    float* A0 = ws.A0;
+    float* A1 = ws.A1;
+    float* Ix = ws.Ix;
+    float* Iy = ws.Iy;
+    float* PROD = ws.PROD;   // per-pixel products (3 or 2 planes of the level size)
+    float* ROWS = ws.ROWS;   // their row sums
+    float* GT = ws.GT;       // structure tensor Gxx, Gxy, Gyy of the level
+    float* u = ws.u;
+    float* v = ws.v;
+    float* nu = ws.nu;
+    float* nv = ws.nv;
     auto run = [&]() -> int {
         OF_HIP(hipMemcpyAsync(P0, dI0, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
         OF_HIP(hipMemcpyAsync(P1, dI1, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -332,7 +446,18 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
             LK_CHECK("k_lk_boxsum");
             hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, 3, wc, 1);
             LK_CHECK("k_lk_boxsum");
-            for (int it = 0; it < iters; it++) {
+            const dim3 fgrid(cdiv(w, 64), cdiv(h, 16));
+            for (int it = 0; it < iters && wc >= 1 && wc <= 4; it++) {
+                // the whole iteration in one kernel, (u, v) -> (nu, nv)
+                if (wc == 1) hipLaunchKernelGGL((k_lk_iter_fused<1>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+                if (wc == 2) hipLaunchKernelGGL((k_lk_iter_fused<2>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+                if (wc == 3) hipLaunchKernelGGL((k_lk_iter_fused<3>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+                if (wc == 4) hipLaunchKernelGGL((k_lk_iter_fused<4>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+                LK_CHECK("k_lk_iter_fused");
+                float* t = u; u = nu; nu = t;
+                t = v; v = nv; nv = t;
+            }
+            for (int it = 0; it < iters && (wc < 1 || wc > 4); it++) {
                 hipLaunchKernelGGL(k_lk_it_products, grid2d(w, h), dim3(256), 0, st, p0, p1, Ix, Iy,
                                    u, v, w, h, PROD);
                 LK_CHECK("k_lk_it_products");
@@ -355,7 +480,6 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
         return OF_OK;
     };
     if (status == OF_OK) status = run();
-    for (float* p : bufs) dev_free(p);
     return status;
 }
 

@@ -58,8 +58,7 @@ struct of_tvl1 {
     float* I1p = nullptr;
     float* ubase = nullptr;     // per scale s at 4*off[s]: [buf0 u1|u2][buf1 u1|u2]
     float* pbase = nullptr;     // [buf0 p11|p12|p21|p22][buf1 ...], buffer stride 4*N0
-    float* I1x = nullptr;
-    float* I1y = nullptr;
+    float* wsrc = nullptr;      // 4*N0: (I1, I1x, I1y, 0) per pixel of the active scale
     float* wx = nullptr;
     float* wy = nullptr;
     float* rhoc = nullptr;
@@ -128,8 +127,7 @@ void release(of_tvl1* h)
     dev_free(h->I1p);
     dev_free(h->ubase);
     dev_free(h->pbase);
-    dev_free(h->I1x);
-    dev_free(h->I1y);
+    dev_free(h->wsrc);
     dev_free(h->wx);
     dev_free(h->wy);
     dev_free(h->rhoc);
@@ -348,33 +346,35 @@ int prepare(of_tvl1* h, const void* src0, const void* src1, bool is_u8, hipStrea
 // `init` (planar [u1|u2] at the coarsest scale, device) as the initial flow,
 // or zero when null (tvl1.cpp:199-200); at most `warps` warpings per scale
 // and `max_iter` iterations per warping.  Waits for the result.
-int solve_levels(of_tvl1* h, const float* init, int warps, int max_iter, float* dst, hipStream_t st)
+int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_iter, float* dst,
+                 hipStream_t st)
 {
     const of_tvl1_params& P = h->prm;
     const size_t N0 = (size_t)h->W * h->H;
     h->run_warps = warps;
     h->run_max = max_iter;
+    // every scale's current buffer back to 0 (the pyramids stay as prepared)
+    OF_TRY(launch_ctl_init(h->ctl, st));
     OF_HIP(hipMemsetAsync(h->d_iters, 0, sizeof(int) * h->h_iters.size(), st));
     {
-        const int s = h->ns - 1;
-        const size_t n2 = 2 * (size_t)h->nx[s] * h->ny[s];
-        float* uc = h->ubase + 4 * h->off[s];   // buffer 0: launch_ctl_init selected it
+        const size_t n2 = 2 * (size_t)h->nx[first] * h->ny[first];
+        float* uc = h->ubase + 4 * h->off[first];   // buffer 0
         if (init) OF_HIP(hipMemcpyAsync(uc, init, sizeof(float) * n2, hipMemcpyDeviceToDevice, st));
         else OF_TRY(launch_fill(uc, n2, 0.0f, st));
     }
     const float invZ = 1 / P.zfactor;
-    for (int s = h->ns - 1; s >= 0; s--) {
+    for (int s = first; s >= 0; s--) {
         const int nx = h->nx[s], ny = h->ny[s];
         const size_t N = (size_t)nx * ny;
         float* I0s = h->I0p + h->off[s];
         float* I1s = h->I1p + h->off[s];
         float* ub = h->ubase + 4 * h->off[s];
         if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s], st));
-        OF_TRY(launch_centered_gradient(I1s, h->I1x, h->I1y, nx, ny, st));
+        OF_TRY(launch_centered_gradient_packed(I1s, h->wsrc, nx, ny, st));
         OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:232-235
         for (int w = 0; w < warps; w++) {
-            OF_TRY(launch_warp_setup(I0s, I1s, h->I1x, h->I1y, ub, ub + 2 * N, h->wx, h->wy,
-                                     h->rhoc, nx, ny, h->ctl, s, st));
+            OF_TRY(launch_warp_setup(I0s, h->wsrc, ub, ub + 2 * N, h->wx, h->wy, h->rhoc, nx, ny,
+                                     h->ctl, s, st));
             OF_TRY(run_warping(h, s, w, st));
         }
         if (s > 0) {
@@ -412,7 +412,7 @@ int solve_levels(of_tvl1* h, const float* init, int warps, int max_iter, float* 
 int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst, hipStream_t st)
 {
     OF_TRY(prepare(h, src0, src1, is_u8, st));
-    return solve_levels(h, nullptr, h->prm.warps, h->prm.max_iterations, dst, st);
+    return solve_levels(h, nullptr, h->ns - 1, h->prm.warps, h->prm.max_iterations, dst, st);
 }
 
 // Launch-shape autotuner: every supported (iterations per launch, rows per
@@ -573,8 +573,7 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
     if (!st) st = dev_alloc(&h->I1p, h->total);
     if (!st) st = dev_alloc(&h->ubase, 4 * h->total);
     if (!st) st = dev_alloc(&h->pbase, 8 * N0);
-    if (!st) st = dev_alloc(&h->I1x, N0);
-    if (!st) st = dev_alloc(&h->I1y, N0);
+    if (!st) st = dev_alloc(&h->wsrc, 4 * N0);
     if (!st) st = dev_alloc(&h->wx, N0);
     if (!st) st = dev_alloc(&h->wy, N0);
     if (!st) st = dev_alloc(&h->rhoc, N0);
@@ -653,15 +652,69 @@ int of_tvl1_scale_images(const of_tvl1* h, int s, const float** dI0s, const floa
     return OF_OK;
 }
 
-int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int warps, int max_iterations, float* dU,
-                           void* stream)
+int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int first_scale, int warps,
+                           int max_iterations, float* dU, void* stream)
 {
     if (!h || !dU || warps < 0 || warps > h->prm.warps || max_iterations < 0) return OF_ERR_ARG;
+    if (first_scale < -1 || first_scale >= h->ns) return OF_ERR_ARG;
     if (!h->prepared) return OF_ERR_STATE;
     hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
+    if (first_scale < 0) first_scale = h->ns - 1;
     if (warps == 0) warps = h->prm.warps;
     if (max_iterations == 0) max_iterations = h->prm.max_iterations;
-    return solve_levels(h, dUinit, warps, max_iterations, dU, st);
+    return solve_levels(h, dUinit, first_scale, warps, max_iterations, dU, st);
+}
+
+int of_tvl1_lift_flow(of_tvl1* h, int from_scale, const float* dUfrom, int to_scale, float* dUto,
+                      void* stream)
+{
+    if (!h || !dUfrom || !dUto || from_scale < 0 || from_scale >= h->ns || to_scale < 0 ||
+        to_scale > from_scale)
+        return OF_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const float invZ = 1 / h->prm.zfactor;
+    const float* src = dUfrom;
+    float* bufs[2] = {h->A0, h->A1};   // N0 floats each: two planes of any scale >= 1
+    if (from_scale == to_scale) {
+        OF_HIP(hipMemcpyAsync(dUto, dUfrom, sizeof(float) * 2 * (size_t)h->nx[to_scale] * h->ny[to_scale],
+                              hipMemcpyDeviceToDevice, st));
+        return OF_OK;
+    }
+    for (int s = from_scale; s > to_scale; s--) {
+        const size_t nc = (size_t)h->nx[s] * h->ny[s], nf = (size_t)h->nx[s - 1] * h->ny[s - 1];
+        float* out = (s - 1 == to_scale) ? dUto : bufs[s & 1];
+        // zoom_in of both components times 1/zfactor, as between solved scales
+        OF_TRY(launch_zoom_in_planes(src, src + nc, out, out + nf, h->nx[s], h->ny[s], h->nx[s - 1],
+                                     h->ny[s - 1], invZ, st));
+        src = out;
+    }
+    return OF_OK;
+}
+
+int of_tvl1_flow_to_scale(of_tvl1* h, const float* dU, int s, float* dUs, void* stream)
+{
+    if (!h || !dU || !dUs || s < 0 || s >= h->ns) return OF_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t N0 = (size_t)h->W * h->H;
+    float scale = 1.0f;
+    for (int c = 0; c < 2; c++) {
+        const float* src = dU + c * N0;
+        float* ping[2] = {h->wx, h->wy};   // N0 floats each
+        for (int l = 0; l < s; l++) {
+            float* out = (l + 1 == s) ? dUs + c * (size_t)h->nx[s] * h->ny[s] : ping[l & 1];
+            // zoom_out (zoom.cpp:20-53): blur, then sample at (j/zf, i/zf)
+            OF_TRY(launch_gaussian(src, h->A0, h->A1, h->nx[l], h->ny[l], h->g_zoom, st));
+            OF_TRY(launch_zoom_out_sample(h->A1, out, h->nx[l], h->ny[l], h->nx[l + 1], h->ny[l + 1],
+                                          h->prm.zfactor, st));
+            src = out;
+        }
+        if (s == 0)
+            OF_HIP(hipMemcpyAsync(dUs + c * N0, src, sizeof(float) * N0, hipMemcpyDeviceToDevice, st));
+    }
+    for (int l = 0; l < s; l++) scale *= h->prm.zfactor;
+    OF_TRY(launch_scale(dUs, 2 * (size_t)h->nx[s] * h->ny[s], scale, st));
+    OF_HIP(hipStreamSynchronize(st));
+    return OF_OK;
 }
 
 int of_tvl1_scale_flow(of_tvl1* h, int s, float* dUs, void* stream)

@@ -262,6 +262,37 @@ __global__ __launch_bounds__(256) void k_centered_gradient(const float* __restri
     dy[(size_t)i * nx + j] = (float)(0.5 * (double)ddy);
 }
 
+// The centred gradient of I1 stored with I1 itself as one float4 per pixel
+// (I1, I1x, I1y, 0): the source of the warping's three bicubic samples.
+__global__ __launch_bounds__(256) void k_centered_gradient_packed(const float* __restrict__ in,
+                                                                  float4* __restrict__ out, int nx,
+                                                                  int ny)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= nx || i >= ny) return;
+    const int jl = j > 0 ? j - 1 : 0, jr = j < nx - 1 ? j + 1 : nx - 1;
+    const int iu = i > 0 ? i - 1 : 0, id = i < ny - 1 ? i + 1 : ny - 1;
+    const float ddx = in[(size_t)i * nx + jr] - in[(size_t)i * nx + jl];
+    const float ddy = in[(size_t)id * nx + j] - in[(size_t)iu * nx + j];
+    out[(size_t)i * nx + j] =
+        make_float4(in[(size_t)i * nx + j], (float)(0.5 * (double)ddx), (float)(0.5 * (double)ddy), 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_pack3(const float* __restrict__ a, const float* __restrict__ b,
+                                               const float* __restrict__ c, float4* __restrict__ out,
+                                               unsigned n)
+{
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = make_float4(a[i], b[i], c[i], 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_scale(float* __restrict__ p, unsigned n, float f)
+{
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = f * p[i];
+}
+
 __global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, unsigned n, float v)
 {
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -272,14 +303,13 @@ __global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, unsigned n,
 // warping set-up (tvl1.cpp:237-254)
 // ---------------------------------------------------------------------------
 
-// Warps I1, I1x, I1y by the current flow with border_out = true (one shared
-// tap geometry, three gathers) and stores I1wx, I1wy and
+// Warps I1, I1x, I1y (interleaved, one float4 per pixel) by the current flow
+// with border_out = true (one shared tap geometry, one 16-byte gather per
+// tap) and stores I1wx, I1wy and
 // rho_c = I1w - I1wx*u1 - I1wy*u2 - I0.  |grad I1w|^2 is recomputed by the
 // iteration kernel with the same expression.  Resets the per-warping control.
 __global__ __launch_bounds__(256) void k_warp_setup(const float* __restrict__ I0,
-                                                    const float* __restrict__ I1,
-                                                    const float* __restrict__ I1x,
-                                                    const float* __restrict__ I1y,
+                                                    const float4* __restrict__ src,
                                                     const float* __restrict__ ub0,
                                                     const float* __restrict__ ub1,
                                                     float* __restrict__ wx, float* __restrict__ wy,
@@ -304,11 +334,7 @@ __global__ __launch_bounds__(256) void k_warp_setup(const float* __restrict__ I0
     const float vv = (float)i + u2;
     const BicubicTaps t = bicubic_taps(uu, vv, nx, ny);
     float w = 0.0f, gx = 0.0f, gy = 0.0f;
-    if (!t.out) {
-        w = bicubic_sample(I1, nx, t);
-        gx = bicubic_sample(I1x, nx, t);
-        gy = bicubic_sample(I1y, nx, t);
-    }
+    if (!t.out) bicubic_sample3(src, nx, t, w, gx, gy);
     wx[p] = gx;
     wy[p] = gy;
     rhoc[p] = ((w - gx * u1) - gy * u2) - I0[p];
@@ -495,6 +521,13 @@ int launch_centered_gradient(const float* in, float* dx, float* dy, int nx, int 
     return OF_OK;
 }
 
+int launch_scale(float* p, size_t n, float f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale, dim3(grid1d(n)), dim3(256), 0, s, p, (unsigned)n, f);
+    OF_LAUNCH_CHECK("k_scale");
+    return OF_OK;
+}
+
 int launch_fill(float* p, size_t n, float v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill, dim3(grid1d(n)), dim3(256), 0, s, p, (unsigned)n, v);
@@ -502,12 +535,29 @@ int launch_fill(float* p, size_t n, float v, hipStream_t s)
     return OF_OK;
 }
 
-int launch_warp_setup(const float* I0, const float* I1, const float* I1x, const float* I1y,
-                      const float* ub0, const float* ub1, float* wx, float* wy, float* rhoc,
-                      int nx, int ny, TvCtl* ctl, int level, hipStream_t s)
+int launch_centered_gradient_packed(const float* in, float* out4, int nx, int ny, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_warp_setup, grid2d(nx, ny), dim3(256), 0, s, I0, I1, I1x, I1y, ub0, ub1,
-                       wx, wy, rhoc, nx, ny, ctl, level);
+    hipLaunchKernelGGL(k_centered_gradient_packed, grid2d(nx, ny), dim3(256), 0, s, in,
+                       reinterpret_cast<float4*>(out4), nx, ny);
+    OF_LAUNCH_CHECK("k_centered_gradient_packed");
+    return OF_OK;
+}
+
+int launch_pack3(const float* a, const float* b, const float* c, float* out4, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack3, dim3(grid1d(n)), dim3(256), 0, s, a, b, c,
+                       reinterpret_cast<float4*>(out4), (unsigned)n);
+    OF_LAUNCH_CHECK("k_pack3");
+    return OF_OK;
+}
+
+int launch_warp_setup(const float* I0, const float* src4, const float* ub0, const float* ub1,
+                      float* wx, float* wy, float* rhoc, int nx, int ny, TvCtl* ctl, int level,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_warp_setup, grid2d(nx, ny), dim3(256), 0, s, I0,
+                       reinterpret_cast<const float4*>(src4), ub0, ub1, wx, wy, rhoc, nx, ny, ctl,
+                       level);
     OF_LAUNCH_CHECK("k_warp_setup");
     return OF_OK;
 }

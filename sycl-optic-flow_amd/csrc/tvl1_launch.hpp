@@ -28,9 +28,13 @@ int launch_zoom_in_planes(const float* a, const float* b, float* oa, float* ob, 
 int launch_centered_gradient(const float* in, float* dx, float* dy, int nx, int ny,
                              hipStream_t s);
 int launch_fill(float* p, size_t n, float v, hipStream_t s);
-int launch_warp_setup(const float* I0, const float* I1, const float* I1x, const float* I1y,
-                      const float* ub0, const float* ub1, float* wx, float* wy, float* rhoc,
-                      int nx, int ny, TvCtl* ctl, int level, hipStream_t s);
+int launch_scale(float* p, size_t n, float f, hipStream_t s);   // p *= f
+// (I1, I1x, I1y, 0) per pixel, 4 floats, 16-byte aligned
+int launch_centered_gradient_packed(const float* in, float* out4, int nx, int ny, hipStream_t s);
+int launch_pack3(const float* a, const float* b, const float* c, float* out4, size_t n, hipStream_t s);
+int launch_warp_setup(const float* I0, const float* src4, const float* ub0, const float* ub1,
+                      float* wx, float* wy, float* rhoc, int nx, int ny, TvCtl* ctl, int level,
+                      hipStream_t s);
 int launch_copy_flow(const float* ub0, const float* ub1, float* out, size_t n2, const TvCtl* ctl,
                      int level, hipStream_t s);
 int launch_energy(const float* I0, const float* I1, const float* u1, const float* u2, int nx,

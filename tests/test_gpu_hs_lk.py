@@ -101,8 +101,11 @@ def test_lk_pyramidal_1280x720_matches_oracle():
     assert np.array_equal(u, ou) and np.array_equal(v, ov)
 
 
-@pytest.mark.parametrize("levels,win,iters", [(1, 5, 3), (2, 9, 4), (4, 7, 2)])
+@pytest.mark.parametrize("levels,win,iters", [(1, 5, 3), (2, 9, 4), (4, 7, 2), (2, 3, 3), (2, 11, 2),
+                                               (1, 1, 2)])
 def test_lk_pyramidal_variants_match_oracle(levels, win, iters):
+    """Windows 3-9 run the fused iteration kernel, 1 and 11 the three-kernel
+    path; both match the oracle bit for bit."""
     I0, I1, _, _ = synth.make_pair(257, 190, 3.0, 22 + levels)
     u, v = lk_pyramidal(I0, I1, levels=levels, window=win, iters=iters)
     ou, ov = O.lk_pyr(I0, I1, levels=levels, win=win, iters=iters)
