@@ -5,8 +5,9 @@ sinusoid + bump motion up to 20 px, noise sigma 1), each solved by the plain
 TV-L1 solver (5 scales, 5 warpings, at most 300 iterations, zero initial
 flow).  The network input is the coarsest scale's (I0, I1, Ix, Iy, It); the
 label is the plain solver's final flow resampled to the coarsest scale
-(of_tvl1_flow_to_scale).  Training: Adam, learning rate 1e-3, mean-squared
-flow error, on the GPU through the library's own forward / backward kernels.
+(of_tvl1_flow_to_scale).  Training: Adam from learning rate 1e-3 (half-cosine
+decay), mean-squared flow error, on the GPU through the library's own
+forward / backward kernels.
 
 Use: the prediction, carried to scale `first_scale` the way the solver
 carries flow between scales, seeds a solve of scales first_scale .. 0 with
@@ -78,6 +79,10 @@ def make_dataset(npairs: int, seed0: int = TRAIN_SEED0, width: int = WIDTH, heig
 
 def train(net: WarmStartNet, X, T, hw, epochs: int, batch: int = 32, lr: float = 1e-3,
           seed: int = 0, log=print):
+    """Adam from learning rate `lr`, decayed along a half cosine to lr/100
+    over the run."""
+    import math
+
     import torch
     hc, wc = hw
     n = X.shape[0]
@@ -85,6 +90,7 @@ def train(net: WarmStartNet, X, T, hw, epochs: int, batch: int = 32, lr: float =
     history = []
     t0 = time.perf_counter()
     steps = 0
+    total = max(1, epochs * (n // batch))
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g).cuda()
         losses = []
@@ -92,7 +98,8 @@ def train(net: WarmStartNet, X, T, hw, epochs: int, batch: int = 32, lr: float =
             idx = perm[k:k + batch]
             xb = X.index_select(0, idx).contiguous()
             tb = T.index_select(0, idx).contiguous()
-            losses.append(net.train_step(xb.data_ptr(), tb.data_ptr(), batch, hc, wc, lr))
+            lr_t = lr * (0.01 + 0.99 * 0.5 * (1.0 + math.cos(math.pi * steps / total)))
+            losses.append(net.train_step(xb.data_ptr(), tb.data_ptr(), batch, hc, wc, lr_t))
             steps += 1
         history.append(float(np.mean(losses)))
         if log:

@@ -93,7 +93,8 @@ __global__ __launch_bounds__(256) void k_load_minmax(const T* __restrict__ a,
     kmax1 = wave_max_u64(kmax1);
     kmin0 = wave_min_u64(kmin0);
     kmin1 = wave_min_u64(kmin1);
-    // one atomic per key per workgroup: the grid is small (see launcher)
+    // per-workgroup keys, reduced by k_minmax_final (no atomics on 4 words
+    // from a thousand workgroups)
     __shared__ unsigned long long sk[4][4];
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -107,10 +108,44 @@ __global__ __launch_bounds__(256) void k_load_minmax(const T* __restrict__ a,
         const int k = threadIdx.x;
         unsigned long long v = sk[k][0];
         for (int w = 1; w < 4; w++) v = (k < 2) ? (sk[k][w] > v ? sk[k][w] : v) : (sk[k][w] < v ? sk[k][w] : v);
-        if (k == 0) atomicMax(&ctl->maxkey[0], v);
-        if (k == 1) atomicMax(&ctl->maxkey[1], v);
-        if (k == 2) atomicMin(&ctl->minkey[0], v);
-        if (k == 3) atomicMin(&ctl->minkey[1], v);
+        ctl->mmpart[k][blockIdx.x] = v;
+    }
+}
+
+// Combines the per-workgroup keys of k_load_minmax (one workgroup).
+__global__ __launch_bounds__(256) void k_minmax_final(TvCtl* ctl, unsigned nblocks)
+{
+    unsigned long long kmax0 = 0ull, kmax1 = 0ull, kmin0 = ~0ull, kmin1 = ~0ull;
+    for (unsigned i = threadIdx.x; i < nblocks; i += 256) {
+        kmax0 = ctl->mmpart[0][i] > kmax0 ? ctl->mmpart[0][i] : kmax0;
+        kmax1 = ctl->mmpart[1][i] > kmax1 ? ctl->mmpart[1][i] : kmax1;
+        kmin0 = ctl->mmpart[2][i] < kmin0 ? ctl->mmpart[2][i] : kmin0;
+        kmin1 = ctl->mmpart[3][i] < kmin1 ? ctl->mmpart[3][i] : kmin1;
+    }
+    kmax0 = wave_max_u64(kmax0);
+    kmax1 = wave_max_u64(kmax1);
+    kmin0 = wave_min_u64(kmin0);
+    kmin1 = wave_min_u64(kmin1);
+    __shared__ unsigned long long sk[4][4];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sk[0][wave] = kmax0;
+        sk[1][wave] = kmax1;
+        sk[2][wave] = kmin0;
+        sk[3][wave] = kmin1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long v[4];
+        for (int k = 0; k < 4; k++) {
+            v[k] = sk[k][0];
+            for (int w = 1; w < 4; w++)
+                v[k] = (k < 2) ? (sk[k][w] > v[k] ? sk[k][w] : v[k]) : (sk[k][w] < v[k] ? sk[k][w] : v[k]);
+        }
+        ctl->maxkey[0] = v[0];
+        ctl->maxkey[1] = v[1];
+        ctl->minkey[0] = v[2];
+        ctl->minkey[1] = v[3];
     }
 }
 
@@ -449,18 +484,22 @@ int launch_ctl_begin(TvCtl* ctl, hipStream_t s)
 int launch_load_minmax_u8(const uint8_t* a, const uint8_t* b, float* fa, float* fb, size_t n,
                           TvCtl* ctl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_load_minmax<uint8_t>, dim3(std::min(grid1d(n), 1024u)), dim3(256), 0, s, a, b, fa, fb,
-                       (unsigned)n, ctl);
+    const unsigned nb = std::min(grid1d(n), (unsigned)kMinMaxBlocks);
+    hipLaunchKernelGGL(k_load_minmax<uint8_t>, dim3(nb), dim3(256), 0, s, a, b, fa, fb, (unsigned)n, ctl);
     OF_LAUNCH_CHECK("k_load_minmax<u8>");
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, ctl, nb);
+    OF_LAUNCH_CHECK("k_minmax_final");
     return OF_OK;
 }
 
 int launch_load_minmax_f32(const float* a, const float* b, float* fa, float* fb, size_t n,
                            TvCtl* ctl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_load_minmax<float>, dim3(std::min(grid1d(n), 1024u)), dim3(256), 0, s, a, b, fa, fb,
-                       (unsigned)n, ctl);
+    const unsigned nb = std::min(grid1d(n), (unsigned)kMinMaxBlocks);
+    hipLaunchKernelGGL(k_load_minmax<float>, dim3(nb), dim3(256), 0, s, a, b, fa, fb, (unsigned)n, ctl);
     OF_LAUNCH_CHECK("k_load_minmax<f32>");
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, ctl, nb);
+    OF_LAUNCH_CHECK("k_minmax_final");
     return OF_OK;
 }
 

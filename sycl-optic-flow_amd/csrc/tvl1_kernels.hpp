@@ -9,6 +9,7 @@
 namespace amdflow {
 
 constexpr int kMaxScales = 32;
+constexpr int kMinMaxBlocks = 1024;   // grid of the frame conversion + min/max kernel
 
 // Device control block of one solve.  Written by the last workgroup of each
 // iteration launch (and by the set-up kernels), read by every workgroup at
@@ -23,6 +24,7 @@ struct TvCtl {
     unsigned wlaunches;         // the same, counted within the current warping
     unsigned long long maxkey[2];   // (|x| bits << 32) | (~index), per image
     unsigned long long minkey[2];   // (|x| bits << 32) | index, per image
+    unsigned long long mmpart[4][kMinMaxBlocks];   // per-workgroup max0, max1, min0, min1
     // grid barrier of the persistent warping kernel: the arrival counter and
     // the generation word that waiting workgroups poll sit on cache lines of
     // their own, away from each other and from the fields above
