@@ -25,6 +25,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sycl-optic-flow_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch of the dominant kernel at this workload, from two
+# rocprofv3 counter runs of this script (FETCH_SIZE, WRITE_SIZE, each scaled
+# by the same counter on a 1 GiB copy run first in the same process:
+# --copy-too); the runs are summarised under profiles/.  Counters cannot be
+# read inside a timed run, so the value is carried here and reported only
+# when the same kernel dominates.
+MEASURED_TRAFFIC = {"k_tvl1_persist<2,8>": 1786528099}
 WORKLOAD = "tvl1_1920x1080"
 SEED = 2024
 
@@ -322,7 +329,7 @@ def main():
     achieved = per_launch_bytes / avg_launch_s / 1e9
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
         "kernel": ("k_tvl1_persist" if dom["persistent"] else "k_tvl1_iter")
                   + f"<{dom['iters_per_launch']},{dom['rows_per_wave']}>",
         "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(avg_launch_s * 1e6, 2),
@@ -330,6 +337,13 @@ def main():
         "kernel_share_of_step": round(dom["milliseconds"] / max(1, args.profile_steps)
                                       / (1e3 * dt / args.steps), 3),
     }
+    # keep "traffic" right after "frac" in the printed object
+    traffic = MEASURED_TRAFFIC.get(roofline["kernel"])
+    roofline = {**{k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
+                "traffic": traffic,
+                **{k: v for k, v in roofline.items() if k not in ("bound", "achieved", "peak", "unit", "frac")}}
+    if traffic is not None:
+        roofline["traffic_note"] = "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
 
     flow = dU.cpu().numpy()
     result = {

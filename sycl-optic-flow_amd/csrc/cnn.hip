@@ -172,6 +172,84 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ X, int B
     }
 }
 
+// The same convolution for CX % 32 == 0 with the operands staged in LDS:
+// a block computes an 8x8-pixel tile of one image (waves take 2 rows each)
+// from a 10x10xCX bf16 input patch and the bf16 weights, both loaded once,
+// so each input pixel and each weight leaves HBM / L2 once per block
+// instead of once per tap and per wave.  MFMA operands come from LDS with
+// 16-byte reads; accumulation, epilogue and results equal k_conv's.
+template <int CX, int NT>
+__global__ __launch_bounds__(256) void k_conv_lds(const float* __restrict__ X, int B, int H, int W,
+                                                  const short* __restrict__ Wp, int N,
+                                                  const float* __restrict__ bias, int sgn, int relu,
+                                                  const float* __restrict__ mask,
+                                                  float* __restrict__ Y)
+{
+    static_assert(CX % 32 == 0, "one 32-wide k-step must stay inside one tap");
+    constexpr int KP = 9 * CX;           // a multiple of 32
+    constexpr int PW = 10, PP = PW * PW;
+    __shared__ __align__(16) short patch[PP * CX];
+    __shared__ __align__(16) short wts[NT * 16 * KP];
+    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const int tile = blockIdx.x;
+    const int b = tile / (tiles_x * tiles_y);
+    const int r = tile - b * tiles_x * tiles_y;
+    const int y0 = (r / tiles_x) * 8, x0 = (r % tiles_x) * 8;
+    // input patch rows y0-1 .. y0+8, columns x0-1 .. x0+8, zero outside
+    for (int e = threadIdx.x; e < PP * (CX / 8); e += 256) {
+        const int pix = e / (CX / 8), c8 = (e - pix * (CX / 8)) * 8;
+        const int yy = y0 - 1 + pix / PW, xx = x0 - 1 + pix % PW;
+        f32x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = 0.0f;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const float4* q = reinterpret_cast<const float4*>(X + (((long long)b * H + yy) * W + xx) * CX + c8);
+            const float4 v0 = q[0], v1 = q[1];
+            v[0] = v0.x; v[1] = v0.y; v[2] = v0.z; v[3] = v0.w;
+            v[4] = v1.x; v[5] = v1.y; v[6] = v1.z; v[7] = v1.w;
+        }
+        *reinterpret_cast<bf16x8*>(patch + pix * CX + c8) = to_bf16x8(v);
+    }
+    for (int e = threadIdx.x; e < NT * 16 * KP / 8; e += 256)
+        reinterpret_cast<s16x8*>(wts)[e] = reinterpret_cast<const s16x8*>(Wp)[e];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int py = wave * 2 + (row >> 3), px = row & 7;   // this lane's A pixel in the tile
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < KP; k0 += 32) {
+        const int tap = k0 / CX, c = k0 - tap * CX + 8 * kg;
+        const int dy = sgn * (tap / 3 - 1), dx = sgn * (tap % 3 - 1);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(patch + ((py + 1 + dy) * PW + (px + 1 + dx)) * CX + c);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wts + (t * 16 + row) * KP + k0 + 8 * kg);
+            acc[t] = mfma32(a, bv, acc[t]);
+        }
+    }
+    // accumulator: column n = t*16 + row, rows (pixels) 4*kg + j of the wave's 16
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int n = t * 16 + row;
+        if (n >= N) continue;
+        const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = 4 * kg + j;
+            const int yy = y0 + wave * 2 + (q >> 3), xx = x0 + (q & 7);
+            if (yy >= H || xx >= W) continue;
+            const long long m = ((long long)b * H + yy) * W + xx;
+            float v = acc[t][j] + bn;
+            if (relu) v = v > 0.0f ? v : 0.0f;
+            if (mask && !(mask[m * N + n] > 0.0f)) v = 0.0f;
+            Y[m * N + n] = v;
+        }
+    }
+}
+
 // Weight-gradient partials.  Block (kt, nt, z) forms the 16x16 tile
 // dW[16*nt ..][16*kt ..] over the pixels [z*chunk, (z+1)*chunk): the four
 // waves take interleaved 32-pixel steps (A = dY^T, B = im2col(X), GEMM-k =
@@ -407,6 +485,17 @@ int launch_conv(const float* X, int B, int H, int W, int CX, const short* Wp, in
     const unsigned blocks = (unsigned)((M + 63) / 64);
     const int NT = N <= 16 ? 1 : 2;
     if (N > 32 || Kpad != (9 * CX + 31) / 32 * 32) return OF_ERR_UNSUPPORTED;
+    if (CX == 32) {   // LDS-staged tiles of 8x8 pixels
+        const unsigned tiles = (unsigned)(B * ((H + 7) / 8) * ((W + 7) / 8));
+        if (NT == 2)
+            hipLaunchKernelGGL((k_conv_lds<32, 2>), dim3(tiles), dim3(256), 0, st, X, B, H, W, Wp, N, bias,
+                               sgn, relu, mask, Y);
+        else
+            hipLaunchKernelGGL((k_conv_lds<32, 1>), dim3(tiles), dim3(256), 0, st, X, B, H, W, Wp, N, bias,
+                               sgn, relu, mask, Y);
+        CNN_CHECK("k_conv_lds");
+        return OF_OK;
+    }
 #define CONV_CASE(cx, nt)                                                                          \
     if (CX == cx && NT == nt) {                                                                    \
         hipLaunchKernelGGL((k_conv<cx, nt>), dim3(blocks), dim3(256), 0, st, X, B, H, W, Wp, Kpad, \
