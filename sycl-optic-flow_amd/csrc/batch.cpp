@@ -68,6 +68,66 @@ int worker_handle(of_tvl1_batch* b, int w, int W, int H, of_tvl1** out)
     return OF_OK;
 }
 
+// The batch itself: workers take pairs longest first until the list is
+// empty or one of them fails.
+int run_batch(of_tvl1_batch* b, int npairs, const int* widths, const int* heights,
+              const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
+              int* iterations_total)
+{
+    // longest first: the largest frames start early and small ones fill in
+    std::vector<int> order(npairs);
+    for (int i = 0; i < npairs; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (long long)widths[x] * heights[x] > (long long)widths[y] * heights[y];
+    });
+    std::atomic<int> next{0};
+    std::atomic<int> status{OF_OK};
+    std::vector<long long> iters(npairs, 0);
+    auto fail = [&](int st) {
+        int expected = OF_OK;
+        status.compare_exchange_strong(expected, st);
+    };
+    auto work = [&](int w) noexcept {
+        try {
+            if (hipSetDevice(b->device) != hipSuccess) {
+                fail(OF_ERR_HIP);
+                return;
+            }
+            while (status.load() == OF_OK) {
+                const int k = next.fetch_add(1);
+                if (k >= npairs) break;
+                const int i = order[k];
+                of_tvl1* h = nullptr;
+                int st = worker_handle(b, w, widths[i], heights[i], &h);
+                if (!st) st = of_tvl1_run_device_u8(h, dI0[i], dI1[i], dU[i], b->wstreams[w]);
+                if (!st && iterations_total) {
+                    std::vector<int> c((size_t)of_tvl1_nscales(h) * b->prm.warps);
+                    of_tvl1_iterations(h, c.data(), (int)c.size());
+                    long long sum = 0;
+                    for (int v : c) sum += v;
+                    iters[i] = sum;
+                }
+                if (st) fail(st);
+            }
+        } catch (...) {
+            fail(OF_ERR_ALLOC);
+        }
+    };
+    const int nw = std::min(b->streams, std::max(npairs, 1));
+    std::vector<std::thread> threads;
+    try {
+        threads.reserve(nw);
+        for (int w = 1; w < nw; w++) threads.emplace_back(work, w);
+    } catch (...) {
+        fail(OF_ERR_ALLOC);   // the workers that did start stop at their next pair
+    }
+    work(0);
+    for (auto& t : threads) t.join();
+    if (iterations_total)
+        for (int i = 0; i < npairs; i++) iterations_total[i] = (int)iters[i];
+    return status.load();
+}
+
 }  // namespace
 
 extern "C" {
@@ -81,8 +141,13 @@ int of_tvl1_batch_create(const of_tvl1_params* params, int streams, of_tvl1_batc
     if (params) b->prm = *params;
     else of_tvl1_default_params(&b->prm);
     b->streams = streams;
-    b->handles.resize(streams);
-    b->wstreams.assign(streams, nullptr);
+    try {
+        b->handles.resize(streams);
+        b->wstreams.assign(streams, nullptr);
+    } catch (...) {
+        delete b;
+        return OF_ERR_ALLOC;
+    }
     bool ok = hipGetDevice(&b->device) == hipSuccess;
     for (int w = 0; w < streams && ok; w++)
         ok = hipStreamCreateWithFlags(&b->wstreams[w], hipStreamNonBlocking) == hipSuccess;
@@ -109,7 +174,11 @@ int of_tvl1_batch_set_shape(of_tvl1_batch* b, long long min_pixels, int iters_pe
                             int rows_per_wave)
 {
     if (!b) return OF_ERR_ARG;
-    b->rules.emplace_back(min_pixels, iters_per_launch, rows_per_wave);
+    try {
+        b->rules.emplace_back(min_pixels, iters_per_launch, rows_per_wave);
+    } catch (...) {
+        return OF_ERR_ALLOC;
+    }
     std::sort(b->rules.begin(), b->rules.end(),
               [](const auto& x, const auto& y) { return std::get<0>(x) > std::get<0>(y); });
     for (auto& m : b->handles)
@@ -126,48 +195,12 @@ int of_tvl1_batch_run_u8(of_tvl1_batch* b, int npairs, const int* widths, const 
         return OF_ERR_ARG;
     for (int i = 0; i < npairs; i++)
         if (!dI0[i] || !dI1[i] || !dU[i] || widths[i] < 2 || heights[i] < 2) return OF_ERR_ARG;
-    // longest first: the largest frames start early and small ones fill in
-    std::vector<int> order(npairs);
-    for (int i = 0; i < npairs; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return (long long)widths[x] * heights[x] > (long long)widths[y] * heights[y];
-    });
-    std::atomic<int> next{0};
-    std::atomic<int> status{OF_OK};
-    std::vector<long long> iters(npairs, 0);
-    auto work = [&](int w) {
-        if (hipSetDevice(b->device) != hipSuccess) {
-            status = OF_ERR_HIP;
-            return;
-        }
-        while (status.load() == OF_OK) {
-            const int k = next.fetch_add(1);
-            if (k >= npairs) break;
-            const int i = order[k];
-            of_tvl1* h = nullptr;
-            int st = worker_handle(b, w, widths[i], heights[i], &h);
-            if (!st) st = of_tvl1_run_device_u8(h, dI0[i], dI1[i], dU[i], b->wstreams[w]);
-            if (!st && iterations_total) {
-                std::vector<int> c((size_t)of_tvl1_nscales(h) * b->prm.warps);
-                of_tvl1_iterations(h, c.data(), (int)c.size());
-                long long s = 0;
-                for (int v : c) s += v;
-                iters[i] = s;
-            }
-            if (st) {
-                int expected = OF_OK;
-                status.compare_exchange_strong(expected, st);
-            }
-        }
-    };
-    const int nw = std::min(b->streams, std::max(npairs, 1));
-    std::vector<std::thread> threads;
-    for (int w = 1; w < nw; w++) threads.emplace_back(work, w);
-    work(0);
-    for (auto& t : threads) t.join();
-    if (iterations_total)
-        for (int i = 0; i < npairs; i++) iterations_total[i] = (int)iters[i];
-    return status.load();
+    // the C ABI never throws: allocation failures become OF_ERR_ALLOC
+    try {
+        return run_batch(b, npairs, widths, heights, dI0, dI1, dU, iterations_total);
+    } catch (...) {
+        return OF_ERR_ALLOC;
+    }
 }
 
 }  // extern "C"

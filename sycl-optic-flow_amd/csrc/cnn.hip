@@ -720,65 +720,67 @@ extern "C" {
 
 int of_cnn_create(int hidden_layers, int channels, unsigned long long seed, of_cnn** out)
 {
-    if (!out) return OF_ERR_ARG;
-    *out = nullptr;
-    if (hidden_layers < 0 || hidden_layers > 16 || channels != 32) return OF_ERR_ARG;
-    of_cnn* h = new (std::nothrow) of_cnn();
-    if (!h) return OF_ERR_ALLOC;
-    h->hidden = hidden_layers;
-    h->channels = channels;
-    std::vector<std::pair<int, int>> dims;
-    dims.push_back({5, channels});
-    for (int i = 0; i < hidden_layers; i++) dims.push_back({channels, channels});
-    dims.push_back({channels, 2});
-    size_t off = 0;
-    for (size_t i = 0; i < dims.size(); i++) {
-        Layer L;
-        L.cin = dims[i].first;
-        L.cout = dims[i].second;
-        L.relu = i + 1 < dims.size();
-        L.woff = off;
-        off += (size_t)L.cout * 9 * L.cin;
-        L.boff = off;
-        off += L.cout;
-        L.Kpad = pad32(9 * L.cin);
-        L.Npad = pad16(L.cout);
-        L.Cpad = pad16(L.cin);
-        L.Kbpad = pad32(9 * L.cout);
-        h->layers.push_back(L);
-    }
-    h->nparams = off;
-    // He-normal initial weights (the head scaled by 0.1), zero biases
-    std::vector<float> w(off, 0.0f);
-    std::mt19937_64 rng(seed);
-    for (const Layer& L : h->layers) {
-        const float sd = std::sqrt(2.0f / (9.0f * (float)L.cin)) * (L.relu ? 1.0f : 0.1f);
-        std::normal_distribution<float> nd(0.0f, sd);
-        for (size_t i = 0; i < (size_t)L.cout * 9 * L.cin; i++) w[L.woff + i] = nd(rng);
-    }
-    int st = dev_alloc(&h->w, off);
-    if (!st) st = dev_alloc(&h->g, off);
-    if (!st) st = dev_alloc(&h->m, off);
-    if (!st) st = dev_alloc(&h->v, off);
-    if (!st) st = dev_alloc(&h->lpart, kLossBlocks);
-    if (!st) st = dev_alloc(&h->loss, 1);
-    for (Layer& L : h->layers) {
-        if (!st) st = dev_alloc(&L.fwd, (size_t)L.Npad * L.Kpad);
-        if (!st) st = dev_alloc(&L.bwd, (size_t)L.Cpad * L.Kbpad);
-    }
-    if (!st && hipMemcpy(h->w, w.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
-    if (!st && hipMemset(h->g, 0, off * sizeof(float)) != hipSuccess) st = OF_ERR_HIP;
-    if (!st && hipMemset(h->m, 0, off * sizeof(float)) != hipSuccess) st = OF_ERR_HIP;
-    if (!st && hipMemset(h->v, 0, off * sizeof(float)) != hipSuccess) st = OF_ERR_HIP;
-    if (!st) st = pack(h, nullptr);
-    if (!st && hipDeviceSynchronize() != hipSuccess) st = OF_ERR_HIP;
-    if (st) {
-        (void)hipGetLastError();
-        release(h);
-        return st;
-    }
-    *out = h;
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!out) return OF_ERR_ARG;
+        *out = nullptr;
+        if (hidden_layers < 0 || hidden_layers > 16 || channels != 32) return OF_ERR_ARG;
+        of_cnn* h = new (std::nothrow) of_cnn();
+        if (!h) return OF_ERR_ALLOC;
+        h->hidden = hidden_layers;
+        h->channels = channels;
+        std::vector<std::pair<int, int>> dims;
+        dims.push_back({5, channels});
+        for (int i = 0; i < hidden_layers; i++) dims.push_back({channels, channels});
+        dims.push_back({channels, 2});
+        size_t off = 0;
+        for (size_t i = 0; i < dims.size(); i++) {
+            Layer L;
+            L.cin = dims[i].first;
+            L.cout = dims[i].second;
+            L.relu = i + 1 < dims.size();
+            L.woff = off;
+            off += (size_t)L.cout * 9 * L.cin;
+            L.boff = off;
+            off += L.cout;
+            L.Kpad = pad32(9 * L.cin);
+            L.Npad = pad16(L.cout);
+            L.Cpad = pad16(L.cin);
+            L.Kbpad = pad32(9 * L.cout);
+            h->layers.push_back(L);
+        }
+        h->nparams = off;
+        // He-normal initial weights (the head scaled by 0.1), zero biases
+        std::vector<float> w(off, 0.0f);
+        std::mt19937_64 rng(seed);
+        for (const Layer& L : h->layers) {
+            const float sd = std::sqrt(2.0f / (9.0f * (float)L.cin)) * (L.relu ? 1.0f : 0.1f);
+            std::normal_distribution<float> nd(0.0f, sd);
+            for (size_t i = 0; i < (size_t)L.cout * 9 * L.cin; i++) w[L.woff + i] = nd(rng);
+        }
+        int st = dev_alloc(&h->w, off);
+        if (!st) st = dev_alloc(&h->g, off);
+        if (!st) st = dev_alloc(&h->m, off);
+        if (!st) st = dev_alloc(&h->v, off);
+        if (!st) st = dev_alloc(&h->lpart, kLossBlocks);
+        if (!st) st = dev_alloc(&h->loss, 1);
+        for (Layer& L : h->layers) {
+            if (!st) st = dev_alloc(&L.fwd, (size_t)L.Npad * L.Kpad);
+            if (!st) st = dev_alloc(&L.bwd, (size_t)L.Cpad * L.Kbpad);
+        }
+        if (!st && hipMemcpy(h->w, w.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
+        if (!st && hipMemset(h->g, 0, off * sizeof(float)) != hipSuccess) st = OF_ERR_HIP;
+        if (!st && hipMemset(h->m, 0, off * sizeof(float)) != hipSuccess) st = OF_ERR_HIP;
+        if (!st && hipMemset(h->v, 0, off * sizeof(float)) != hipSuccess) st = OF_ERR_HIP;
+        if (!st) st = pack(h, nullptr);
+        if (!st && hipDeviceSynchronize() != hipSuccess) st = OF_ERR_HIP;
+        if (st) {
+            (void)hipGetLastError();
+            release(h);
+            return st;
+        }
+        *out = h;
+        return OF_OK;
+    });
 }
 
 void of_cnn_destroy(of_cnn* h) { release(h); }
@@ -817,61 +819,69 @@ int of_cnn_features(const float* dI0s, const float* dI1s, int H, int W, float* d
 
 int of_cnn_forward(of_cnn* h, const float* dX, int B, int H, int W, float* dY, void* stream)
 {
-    if (!dims_ok(h, dX, dY, B, H, W)) return OF_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;
-    OF_TRY(ensure_ws(h, (long long)B * H * W));
-    OF_TRY(forward(h, dX, B, H, W, st));
-    OF_HIP(hipMemcpyAsync(dY, h->act.back(), (size_t)B * H * W * 2 * sizeof(float),
-                          hipMemcpyDeviceToDevice, st));
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!dims_ok(h, dX, dY, B, H, W)) return OF_ERR_ARG;
+        hipStream_t st = (hipStream_t)stream;
+        OF_TRY(ensure_ws(h, (long long)B * H * W));
+        OF_TRY(forward(h, dX, B, H, W, st));
+        OF_HIP(hipMemcpyAsync(dY, h->act.back(), (size_t)B * H * W * 2 * sizeof(float),
+                              hipMemcpyDeviceToDevice, st));
+        return OF_OK;
+    });
 }
 
 int of_cnn_predict(of_cnn* h, const float* dI0s, const float* dI1s, int H, int W, float* dU, void* stream)
 {
-    if (!dims_ok(h, dI0s, dI1s, 1, H, W) || !dU) return OF_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;
-    const long long n = (long long)H * W;
-    OF_TRY(ensure_in(h, n));
-    OF_TRY(ensure_ws(h, n));
-    OF_TRY(of_cnn_features(dI0s, dI1s, H, W, h->xin, st));
-    OF_TRY(forward(h, h->xin, 1, H, W, st));
-    hipLaunchKernelGGL(k_to_planar, dim3(blocks256(n)), dim3(256), 0, st, h->act.back(), (int)n, dU);
-    CNN_CHECK("k_to_planar");
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!dims_ok(h, dI0s, dI1s, 1, H, W) || !dU) return OF_ERR_ARG;
+        hipStream_t st = (hipStream_t)stream;
+        const long long n = (long long)H * W;
+        OF_TRY(ensure_in(h, n));
+        OF_TRY(ensure_ws(h, n));
+        OF_TRY(of_cnn_features(dI0s, dI1s, H, W, h->xin, st));
+        OF_TRY(forward(h, h->xin, 1, H, W, st));
+        hipLaunchKernelGGL(k_to_planar, dim3(blocks256(n)), dim3(256), 0, st, h->act.back(), (int)n, dU);
+        CNN_CHECK("k_to_planar");
+        return OF_OK;
+    });
 }
 
 int of_cnn_gradients(of_cnn* h, const float* dX, const float* dT, int B, int H, int W, float* host_grads,
                      double* loss)
 {
-    if (!dims_ok(h, dX, dT, B, H, W)) return OF_ERR_ARG;
-    OF_TRY(ensure_ws(h, (long long)B * H * W));
-    OF_TRY(forward(h, dX, B, H, W, nullptr));
-    OF_TRY(loss_grad(h, dT, B, H, W, nullptr));
-    OF_TRY(backward(h, dX, B, H, W, nullptr));
-    OF_HIP(hipDeviceSynchronize());
-    if (host_grads) OF_HIP(hipMemcpy(host_grads, h->g, h->nparams * sizeof(float), hipMemcpyDeviceToHost));
-    if (loss) OF_HIP(hipMemcpy(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost));
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!dims_ok(h, dX, dT, B, H, W)) return OF_ERR_ARG;
+        OF_TRY(ensure_ws(h, (long long)B * H * W));
+        OF_TRY(forward(h, dX, B, H, W, nullptr));
+        OF_TRY(loss_grad(h, dT, B, H, W, nullptr));
+        OF_TRY(backward(h, dX, B, H, W, nullptr));
+        OF_HIP(hipDeviceSynchronize());
+        if (host_grads) OF_HIP(hipMemcpy(host_grads, h->g, h->nparams * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) OF_HIP(hipMemcpy(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost));
+        return OF_OK;
+    });
 }
 
 int of_cnn_train_step(of_cnn* h, const float* dX, const float* dT, int B, int H, int W, float lr,
                       double* loss)
 {
-    if (!dims_ok(h, dX, dT, B, H, W) || !(lr > 0.0f)) return OF_ERR_ARG;
-    OF_TRY(ensure_ws(h, (long long)B * H * W));
-    OF_TRY(forward(h, dX, B, H, W, nullptr));
-    OF_TRY(loss_grad(h, dT, B, H, W, nullptr));
-    OF_TRY(backward(h, dX, B, H, W, nullptr));
-    h->step++;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - std::pow(b1, (float)h->step), c2 = 1.0f - std::pow(b2, (float)h->step);
-    hipLaunchKernelGGL(k_adam, dim3(blocks256((long long)h->nparams)), dim3(256), 0, nullptr, h->w, h->g,
-                       h->m, h->v, (int)h->nparams, lr, b1, b2, 1e-8f, c1, c2);
-    CNN_CHECK("k_adam");
-    OF_TRY(pack(h, nullptr));
-    if (loss) OF_HIP(hipMemcpy(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost));
-    else OF_HIP(hipDeviceSynchronize());
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!dims_ok(h, dX, dT, B, H, W) || !(lr > 0.0f)) return OF_ERR_ARG;
+        OF_TRY(ensure_ws(h, (long long)B * H * W));
+        OF_TRY(forward(h, dX, B, H, W, nullptr));
+        OF_TRY(loss_grad(h, dT, B, H, W, nullptr));
+        OF_TRY(backward(h, dX, B, H, W, nullptr));
+        h->step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, (float)h->step), c2 = 1.0f - std::pow(b2, (float)h->step);
+        hipLaunchKernelGGL(k_adam, dim3(blocks256((long long)h->nparams)), dim3(256), 0, nullptr, h->w, h->g,
+                           h->m, h->v, (int)h->nparams, lr, b1, b2, 1e-8f, c1, c2);
+        CNN_CHECK("k_adam");
+        OF_TRY(pack(h, nullptr));
+        if (loss) OF_HIP(hipMemcpy(loss, h->loss, sizeof(double), hipMemcpyDeviceToHost));
+        else OF_HIP(hipDeviceSynchronize());
+        return OF_OK;
+    });
 }
 
 // Warm-started solve of one pair: prepare the pyramids, predict the
@@ -885,43 +895,45 @@ int of_tvl1_warm_run_device_u8(of_tvl1* tv, of_cnn* net, const uint8_t* dI0, con
                                int first_scale, int warps, int max_iterations, float* dU,
                                int* fell_back, double* energies, void* stream)
 {
-    if (!tv || !net || !dI0 || !dI1 || !dU || warps < 0 || max_iterations < 0) return OF_ERR_ARG;
-    const int ns = of_tvl1_nscales(tv);
-    if (first_scale < -1 || first_scale >= ns) return OF_ERR_ARG;
-    if (first_scale < 0) first_scale = ns - 1;
-    int nx = 0, ny = 0, fx = 0, fy = 0;
-    OF_TRY(of_tvl1_scale_size(tv, ns - 1, &nx, &ny));
-    OF_TRY(of_tvl1_scale_size(tv, first_scale, &fx, &fy));
-    const long long n = (long long)nx * ny, nf = (long long)fx * fy;
-    OF_TRY(ensure_in(net, n));
-    if (2 * nf > net->cap_lift) {
-        OF_HIP(hipDeviceSynchronize());
-        dev_free(net->lift);
-        dev_free(net->zero);
-        net->cap_lift = 0;
-        OF_TRY(dev_alloc(&net->lift, 2 * (size_t)nf));
-        OF_TRY(dev_alloc(&net->zero, 2 * (size_t)nf));
-        net->cap_lift = 2 * nf;
-    }
-    hipStream_t st = (hipStream_t)stream;
-    OF_TRY(of_tvl1_prepare_device_u8(tv, dI0, dI1, stream));
-    const float* I0c = nullptr;
-    const float* I1c = nullptr;
-    OF_TRY(of_tvl1_scale_images(tv, ns - 1, &I0c, &I1c));
-    OF_TRY(of_cnn_predict(net, I0c, I1c, ny, nx, net->pred, stream));
-    OF_TRY(of_tvl1_lift_flow(tv, ns - 1, net->pred, first_scale, net->lift, stream));
-    OF_HIP(hipMemsetAsync(net->zero, 0, sizeof(float) * 2 * (size_t)nf, st));
-    double ep[3], ez[3];
-    OF_TRY(of_tvl1_energy(tv, first_scale, net->lift, ep, stream));
-    OF_TRY(of_tvl1_energy(tv, first_scale, net->zero, ez, stream));
-    if (energies) {
-        energies[0] = ep[0];
-        energies[1] = ez[0];
-    }
-    const bool keep = ep[0] <= ez[0];
-    if (fell_back) *fell_back = keep ? 0 : 1;
-    if (keep) return of_tvl1_solve_prepared(tv, net->lift, first_scale, warps, max_iterations, dU, stream);
-    return of_tvl1_solve_prepared(tv, nullptr, -1, 0, 0, dU, stream);
+    return guarded([&]() -> int {
+        if (!tv || !net || !dI0 || !dI1 || !dU || warps < 0 || max_iterations < 0) return OF_ERR_ARG;
+        const int ns = of_tvl1_nscales(tv);
+        if (first_scale < -1 || first_scale >= ns) return OF_ERR_ARG;
+        if (first_scale < 0) first_scale = ns - 1;
+        int nx = 0, ny = 0, fx = 0, fy = 0;
+        OF_TRY(of_tvl1_scale_size(tv, ns - 1, &nx, &ny));
+        OF_TRY(of_tvl1_scale_size(tv, first_scale, &fx, &fy));
+        const long long n = (long long)nx * ny, nf = (long long)fx * fy;
+        OF_TRY(ensure_in(net, n));
+        if (2 * nf > net->cap_lift) {
+            OF_HIP(hipDeviceSynchronize());
+            dev_free(net->lift);
+            dev_free(net->zero);
+            net->cap_lift = 0;
+            OF_TRY(dev_alloc(&net->lift, 2 * (size_t)nf));
+            OF_TRY(dev_alloc(&net->zero, 2 * (size_t)nf));
+            net->cap_lift = 2 * nf;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        OF_TRY(of_tvl1_prepare_device_u8(tv, dI0, dI1, stream));
+        const float* I0c = nullptr;
+        const float* I1c = nullptr;
+        OF_TRY(of_tvl1_scale_images(tv, ns - 1, &I0c, &I1c));
+        OF_TRY(of_cnn_predict(net, I0c, I1c, ny, nx, net->pred, stream));
+        OF_TRY(of_tvl1_lift_flow(tv, ns - 1, net->pred, first_scale, net->lift, stream));
+        OF_HIP(hipMemsetAsync(net->zero, 0, sizeof(float) * 2 * (size_t)nf, st));
+        double ep[3], ez[3];
+        OF_TRY(of_tvl1_energy(tv, first_scale, net->lift, ep, stream));
+        OF_TRY(of_tvl1_energy(tv, first_scale, net->zero, ez, stream));
+        if (energies) {
+            energies[0] = ep[0];
+            energies[1] = ez[0];
+        }
+        const bool keep = ep[0] <= ez[0];
+        if (fell_back) *fell_back = keep ? 0 : 1;
+        if (keep) return of_tvl1_solve_prepared(tv, net->lift, first_scale, warps, max_iterations, dU, stream);
+        return of_tvl1_solve_prepared(tv, nullptr, -1, 0, 0, dU, stream);
+    });
 }
 
 // ---- test entry point: MFMA operand layout through k_conv ----
@@ -931,27 +943,29 @@ int of_tvl1_warm_run_device_u8(of_tvl1* tv, of_cnn* net, const uint8_t* dI0, con
 // non-zero tap is the centre one.
 int of_dbg_cnn_mfma(const float* A, const float* Bm, float* D)
 {
-    if (!A || !Bm || !D) return OF_ERR_ARG;
-    float* dX = nullptr;
-    float* dY = nullptr;
-    short* dW = nullptr;
-    const int Kpad = 160;
-    int st = dev_alloc(&dX, 256);
-    if (!st) st = dev_alloc(&dY, 256);
-    if (!st) st = dev_alloc(&dW, 16 * Kpad);
-    std::vector<short> w(16 * Kpad, 0);
-    for (int n = 0; n < 16; n++)
-        for (int c = 0; c < 16; c++) w[n * Kpad + 4 * 16 + c] = to_bf16(Bm[c * 16 + n]);
-    if (!st && hipMemcpy(dX, A, 256 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
-    if (!st && hipMemcpy(dW, w.data(), w.size() * sizeof(short), hipMemcpyHostToDevice) != hipSuccess)
-        st = OF_ERR_HIP;
-    if (!st) st = launch_conv(dX, 1, 1, 16, 16, dW, Kpad, 16, nullptr, 1, 0, nullptr, dY, nullptr);
-    if (!st && hipDeviceSynchronize() != hipSuccess) st = OF_ERR_HIP;
-    if (!st && hipMemcpy(D, dY, 256 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) st = OF_ERR_HIP;
-    dev_free(dX);
-    dev_free(dY);
-    dev_free(dW);
-    return st;
+    return guarded([&]() -> int {
+        if (!A || !Bm || !D) return OF_ERR_ARG;
+        float* dX = nullptr;
+        float* dY = nullptr;
+        short* dW = nullptr;
+        const int Kpad = 160;
+        int st = dev_alloc(&dX, 256);
+        if (!st) st = dev_alloc(&dY, 256);
+        if (!st) st = dev_alloc(&dW, 16 * Kpad);
+        std::vector<short> w(16 * Kpad, 0);
+        for (int n = 0; n < 16; n++)
+            for (int c = 0; c < 16; c++) w[n * Kpad + 4 * 16 + c] = to_bf16(Bm[c * 16 + n]);
+        if (!st && hipMemcpy(dX, A, 256 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
+        if (!st && hipMemcpy(dW, w.data(), w.size() * sizeof(short), hipMemcpyHostToDevice) != hipSuccess)
+            st = OF_ERR_HIP;
+        if (!st) st = launch_conv(dX, 1, 1, 16, 16, dW, Kpad, 16, nullptr, 1, 0, nullptr, dY, nullptr);
+        if (!st && hipDeviceSynchronize() != hipSuccess) st = OF_ERR_HIP;
+        if (!st && hipMemcpy(D, dY, 256 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) st = OF_ERR_HIP;
+        dev_free(dX);
+        dev_free(dY);
+        dev_free(dW);
+        return st;
+    });
 }
 
 }  // extern "C"

@@ -28,6 +28,19 @@ namespace amdflow {
 
 void record_hip_error(hipError_t e, const char* what, const char* file, int line);
 
+// Runs an entry point's body; the C ABI never lets an exception out (the
+// library throws none itself, but its std containers can report a failed
+// allocation that way), so any exception becomes OF_ERR_ALLOC.
+template <typename F>
+int guarded(F&& body) noexcept
+{
+    try {
+        return body();
+    } catch (...) {
+        return OF_ERR_ALLOC;
+    }
+}
+
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // Device allocation that reports failure as a status code.

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "amdflow.h"
@@ -111,59 +112,64 @@ int of_flow_to_color(const float* U, int width, int height, uint8_t* bgr)
 
 int of_flo_write(const char* path, const float* U, int width, int height)
 {
-    if (!path || !U || width < 1 || height < 1) return OF_ERR_ARG;
-    FILE* f = std::fopen(path, "wb");
-    if (!f) return OF_ERR_ARG;
-    const size_t n = (size_t)width * height;
-    const int32_t wh[2] = {width, height};
-    bool ok = std::fwrite(&kFloTag, 4, 1, f) == 1 && std::fwrite(wh, 4, 2, f) == 2;
-    std::vector<float> row(2 * (size_t)width);
-    for (int y = 0; y < height && ok; y++) {
-        for (int x = 0; x < width; x++) {
-            row[2 * x] = U[(size_t)y * width + x];
-            row[2 * x + 1] = U[n + (size_t)y * width + x];
+    return guarded([&]() -> int {
+        if (!path || !U || width < 1 || height < 1) return OF_ERR_ARG;
+        std::vector<float> row(2 * (size_t)width);
+        FILE* f = std::fopen(path, "wb");
+        if (!f) return OF_ERR_ARG;
+        const size_t n = (size_t)width * height;
+        const int32_t wh[2] = {width, height};
+        bool ok = std::fwrite(&kFloTag, 4, 1, f) == 1 && std::fwrite(wh, 4, 2, f) == 2;
+        for (int y = 0; y < height && ok; y++) {
+            for (int x = 0; x < width; x++) {
+                row[2 * x] = U[(size_t)y * width + x];
+                row[2 * x + 1] = U[n + (size_t)y * width + x];
+            }
+            ok = std::fwrite(row.data(), sizeof(float), row.size(), f) == row.size();
         }
-        ok = std::fwrite(row.data(), sizeof(float), row.size(), f) == row.size();
-    }
-    ok = (std::fclose(f) == 0) && ok;
-    return ok ? OF_OK : OF_ERR_ARG;
+        ok = (std::fclose(f) == 0) && ok;
+        return ok ? OF_OK : OF_ERR_ARG;
+    });
 }
 
 int of_flo_read(const char* path, float* U, long long cap, int* width, int* height)
 {
-    if (!path || !width || !height) return OF_ERR_ARG;
-    FILE* f = std::fopen(path, "rb");
-    if (!f) return OF_ERR_ARG;
-    float tag = 0.0f;
-    int32_t wh[2] = {0, 0};
-    int st = OF_OK;
-    if (std::fread(&tag, 4, 1, f) != 1 || tag != kFloTag || std::fread(wh, 4, 2, f) != 2 || wh[0] < 1 ||
-        wh[1] < 1 || (long long)wh[0] * wh[1] > (1ll << 31))
-        st = OF_ERR_ARG;
-    if (!st) {
-        *width = wh[0];
-        *height = wh[1];
-        const size_t n = (size_t)wh[0] * wh[1];
-        if (U) {
-            if ((long long)(2 * n) > cap) {
-                st = OF_ERR_ARG;
-            } else {
-                std::vector<float> row(2 * (size_t)wh[0]);
-                for (int y = 0; y < wh[1] && !st; y++) {
-                    if (std::fread(row.data(), sizeof(float), row.size(), f) != row.size()) {
-                        st = OF_ERR_ARG;
-                        break;
-                    }
-                    for (int x = 0; x < wh[0]; x++) {
-                        U[(size_t)y * wh[0] + x] = row[2 * x];
-                        U[n + (size_t)y * wh[0] + x] = row[2 * x + 1];
+    return guarded([&]() -> int {
+        if (!path || !width || !height) return OF_ERR_ARG;
+        // the handle is closed on every way out, a failed allocation included
+        std::unique_ptr<FILE, int (*)(FILE*)> file(std::fopen(path, "rb"), &std::fclose);
+        FILE* f = file.get();
+        if (!f) return OF_ERR_ARG;
+        float tag = 0.0f;
+        int32_t wh[2] = {0, 0};
+        int st = OF_OK;
+        if (std::fread(&tag, 4, 1, f) != 1 || tag != kFloTag || std::fread(wh, 4, 2, f) != 2 || wh[0] < 1 ||
+            wh[1] < 1 || (long long)wh[0] * wh[1] > (1ll << 31))
+            st = OF_ERR_ARG;
+        if (!st) {
+            *width = wh[0];
+            *height = wh[1];
+            const size_t n = (size_t)wh[0] * wh[1];
+            if (U) {
+                if ((long long)(2 * n) > cap) {
+                    st = OF_ERR_ARG;
+                } else {
+                    std::vector<float> row(2 * (size_t)wh[0]);
+                    for (int y = 0; y < wh[1] && !st; y++) {
+                        if (std::fread(row.data(), sizeof(float), row.size(), f) != row.size()) {
+                            st = OF_ERR_ARG;
+                            break;
+                        }
+                        for (int x = 0; x < wh[0]; x++) {
+                            U[(size_t)y * wh[0] + x] = row[2 * x];
+                            U[n + (size_t)y * wh[0] + x] = row[2 * x + 1];
+                        }
                     }
                 }
             }
         }
-    }
-    std::fclose(f);
-    return st;
+        return st;
+    });
 }
 
 }  // extern "C"

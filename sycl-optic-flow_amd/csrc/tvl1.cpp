@@ -510,104 +510,106 @@ void of_tvl1_default_params(of_tvl1_params* p)
 
 int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1** out)
 {
-    if (!out) return OF_ERR_ARG;
-    *out = nullptr;
-    of_tvl1_params p;
-    if (params) p = *params;
-    else of_tvl1_default_params(&p);
-    OF_TRY(validate(p, width, height));
-
-    // scale count, tvl1.cpp:84-85, clamped to >= 1
-    const float logz = std::log(1.0f / p.zfactor);
-    const double q = std::log(std::hypot((double)width, (double)height) / 16.0) / (double)logz;
-    const float Nf = (float)(1.0 + q);
-    int ns = (int)((Nf < (float)p.nscales) ? Nf : (float)p.nscales);
-    if (ns < 1) ns = 1;
-    if (ns > kMaxScales) ns = kMaxScales;
-
-    of_tvl1* h = new (std::nothrow) of_tvl1();
-    if (!h) return OF_ERR_ALLOC;
-    h->prm = p;
-    h->W = width;
-    h->H = height;
-    h->ns = ns;
-    h->nx[0] = width;
-    h->ny[0] = height;
-    for (int s = 1; s < ns; s++) zoom_size(h->nx[s - 1], h->ny[s - 1], p.zfactor, &h->nx[s], &h->ny[s]);
-    size_t off = 0;
-    for (int s = 0; s < ns; s++) {
-        if (h->nx[s] < 2 || h->ny[s] < 2) { delete h; return OF_ERR_ARG; }
-        h->off[s] = off;
-        off += (size_t)h->nx[s] * h->ny[s];
-    }
-    h->total = off;
-
-    // Gaussian taps: pre-smoothing sigma 0.8f (tvl1.hpp:7) and the zoom_out
-    // sigma 0.6f*sqrt(1/zf^2 - 1) rounded to float (zoom.cpp:38).
-    int st = gaussian_taps((double)0.8f, &h->g_pre);
-    const float zsig = (float)((double)0.6f * std::sqrt(1.0 / (double)(p.zfactor * p.zfactor) - 1.0));
-    if (!st) st = gaussian_taps((double)zsig, &h->g_zoom);
-    if (!st && (h->g_pre.size > width || h->g_pre.size > height)) st = OF_ERR_SIGMA;
-    for (int s = 0; s + 1 < ns && !st; s++)
-        if (h->g_zoom.size > h->nx[s] || h->g_zoom.size > h->ny[s]) st = OF_ERR_SIGMA;
-    if (st) { delete h; return st; }
-
-    const size_t N0 = (size_t)width * height;
-    size_t maxpart = 1;
-    const int Ks[] = {1, 2, 4, 8}, Rs[] = {2, 4, 8};
-    for (int K : Ks)
-        for (int R : Rs)
-            if (iter_config_ok(K, R)) {
-                unsigned gx, gy;
-                iter_grid(K, R, width, height, &gx, &gy);
-                maxpart = std::max(maxpart, (size_t)gx * gy * K);
-            }
-
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-        h->stream = nullptr;
-        release(h);
-        return OF_ERR_HIP;
-    }
-    st = OF_OK;
-    if (!st) st = dev_alloc(&h->I0p, h->total);
-    if (!st) st = dev_alloc(&h->I1p, h->total);
-    if (!st) st = dev_alloc(&h->ubase, 4 * h->total);
-    if (!st) st = dev_alloc(&h->pbase, 8 * N0);
-    if (!st) st = dev_alloc(&h->wsrc, 4 * N0);
-    if (!st) st = dev_alloc(&h->wx, N0);
-    if (!st) st = dev_alloc(&h->wy, N0);
-    if (!st) st = dev_alloc(&h->rhoc, N0);
-    if (!st) st = dev_alloc(&h->A0, N0);
-    if (!st) st = dev_alloc(&h->A1, N0);
-    if (!st) st = dev_alloc(&h->in8, 2 * N0);
-    if (!st) st = dev_alloc(&h->inf, 2 * N0);
-    if (!st) st = dev_alloc(&h->prev, N0);
-    if (!st) st = dev_alloc(&h->dflow, 2 * N0);
-    if (!st) st = dev_alloc(&h->ctl, 1);
-    // x2: the persistent kernel double-buffers its partials by chunk parity
-    if (!st) st = dev_alloc(&h->partials, 2 * maxpart);
-    if (!st) st = dev_alloc(&h->energy, 2 * kEnergyBlocks + 3);
-    if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
-    if (!st && hipHostMalloc((void**)&h->hflow, sizeof(float) * 2 * N0, hipHostMallocDefault) != hipSuccess) {
-        h->hflow = nullptr;
-        st = OF_ERR_ALLOC;
-    }
-    if (!st && hipHostMalloc((void**)&h->hstatus, sizeof(unsigned long long),
-                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-        h->hstatus = nullptr;
-        st = OF_ERR_ALLOC;
-    }
-    if (st) { (void)hipGetLastError(); release(h); return st; }
-    *h->hstatus = 0ull;
-    std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
-    h->h_iters.assign((size_t)ns * p.warps, 0);
-    if (hipMemsetAsync(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess) {
-        release(h);
-        return OF_ERR_HIP;
-    }
-    *out = h;
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!out) return OF_ERR_ARG;
+        *out = nullptr;
+        of_tvl1_params p;
+        if (params) p = *params;
+        else of_tvl1_default_params(&p);
+        OF_TRY(validate(p, width, height));
+    
+        // scale count, tvl1.cpp:84-85, clamped to >= 1
+        const float logz = std::log(1.0f / p.zfactor);
+        const double q = std::log(std::hypot((double)width, (double)height) / 16.0) / (double)logz;
+        const float Nf = (float)(1.0 + q);
+        int ns = (int)((Nf < (float)p.nscales) ? Nf : (float)p.nscales);
+        if (ns < 1) ns = 1;
+        if (ns > kMaxScales) ns = kMaxScales;
+    
+        of_tvl1* h = new (std::nothrow) of_tvl1();
+        if (!h) return OF_ERR_ALLOC;
+        h->prm = p;
+        h->W = width;
+        h->H = height;
+        h->ns = ns;
+        h->nx[0] = width;
+        h->ny[0] = height;
+        for (int s = 1; s < ns; s++) zoom_size(h->nx[s - 1], h->ny[s - 1], p.zfactor, &h->nx[s], &h->ny[s]);
+        size_t off = 0;
+        for (int s = 0; s < ns; s++) {
+            if (h->nx[s] < 2 || h->ny[s] < 2) { delete h; return OF_ERR_ARG; }
+            h->off[s] = off;
+            off += (size_t)h->nx[s] * h->ny[s];
+        }
+        h->total = off;
+    
+        // Gaussian taps: pre-smoothing sigma 0.8f (tvl1.hpp:7) and the zoom_out
+        // sigma 0.6f*sqrt(1/zf^2 - 1) rounded to float (zoom.cpp:38).
+        int st = gaussian_taps((double)0.8f, &h->g_pre);
+        const float zsig = (float)((double)0.6f * std::sqrt(1.0 / (double)(p.zfactor * p.zfactor) - 1.0));
+        if (!st) st = gaussian_taps((double)zsig, &h->g_zoom);
+        if (!st && (h->g_pre.size > width || h->g_pre.size > height)) st = OF_ERR_SIGMA;
+        for (int s = 0; s + 1 < ns && !st; s++)
+            if (h->g_zoom.size > h->nx[s] || h->g_zoom.size > h->ny[s]) st = OF_ERR_SIGMA;
+        if (st) { delete h; return st; }
+    
+        const size_t N0 = (size_t)width * height;
+        size_t maxpart = 1;
+        const int Ks[] = {1, 2, 4, 8}, Rs[] = {2, 4, 8};
+        for (int K : Ks)
+            for (int R : Rs)
+                if (iter_config_ok(K, R)) {
+                    unsigned gx, gy;
+                    iter_grid(K, R, width, height, &gx, &gy);
+                    maxpart = std::max(maxpart, (size_t)gx * gy * K);
+                }
+    
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+            h->stream = nullptr;
+            release(h);
+            return OF_ERR_HIP;
+        }
+        st = OF_OK;
+        if (!st) st = dev_alloc(&h->I0p, h->total);
+        if (!st) st = dev_alloc(&h->I1p, h->total);
+        if (!st) st = dev_alloc(&h->ubase, 4 * h->total);
+        if (!st) st = dev_alloc(&h->pbase, 8 * N0);
+        if (!st) st = dev_alloc(&h->wsrc, 4 * N0);
+        if (!st) st = dev_alloc(&h->wx, N0);
+        if (!st) st = dev_alloc(&h->wy, N0);
+        if (!st) st = dev_alloc(&h->rhoc, N0);
+        if (!st) st = dev_alloc(&h->A0, N0);
+        if (!st) st = dev_alloc(&h->A1, N0);
+        if (!st) st = dev_alloc(&h->in8, 2 * N0);
+        if (!st) st = dev_alloc(&h->inf, 2 * N0);
+        if (!st) st = dev_alloc(&h->prev, N0);
+        if (!st) st = dev_alloc(&h->dflow, 2 * N0);
+        if (!st) st = dev_alloc(&h->ctl, 1);
+        // x2: the persistent kernel double-buffers its partials by chunk parity
+        if (!st) st = dev_alloc(&h->partials, 2 * maxpart);
+        if (!st) st = dev_alloc(&h->energy, 2 * kEnergyBlocks + 3);
+        if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
+        if (!st && hipHostMalloc((void**)&h->hflow, sizeof(float) * 2 * N0, hipHostMallocDefault) != hipSuccess) {
+            h->hflow = nullptr;
+            st = OF_ERR_ALLOC;
+        }
+        if (!st && hipHostMalloc((void**)&h->hstatus, sizeof(unsigned long long),
+                                 hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+            h->hstatus = nullptr;
+            st = OF_ERR_ALLOC;
+        }
+        if (st) { (void)hipGetLastError(); release(h); return st; }
+        *h->hstatus = 0ull;
+        std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
+        h->h_iters.assign((size_t)ns * p.warps, 0);
+        if (hipMemsetAsync(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            release(h);
+            return OF_ERR_HIP;
+        }
+        *out = h;
+        return OF_OK;
+    });
 }
 
 void of_tvl1_destroy(of_tvl1* h) { release(h); }
@@ -624,23 +626,29 @@ int of_tvl1_scale_size(const of_tvl1* h, int s, int* nx, int* ny)
 
 int of_tvl1_run_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, float* dU, void* stream)
 {
-    if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
-    return solve(h, dI0, dI1, true, dU, st);
+    return guarded([&]() -> int {
+        if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
+        hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
+        return solve(h, dI0, dI1, true, dU, st);
+    });
 }
 
 int of_tvl1_run_device_f32(of_tvl1* h, const float* dI0, const float* dI1, float* dU, void* stream)
 {
-    if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
-    return solve(h, dI0, dI1, false, dU, st);
+    return guarded([&]() -> int {
+        if (!h || !dI0 || !dI1 || !dU) return OF_ERR_ARG;
+        hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
+        return solve(h, dI0, dI1, false, dU, st);
+    });
 }
 
 int of_tvl1_prepare_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, void* stream)
 {
-    if (!h || !dI0 || !dI1) return OF_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
-    return prepare(h, dI0, dI1, true, st);
+    return guarded([&]() -> int {
+        if (!h || !dI0 || !dI1) return OF_ERR_ARG;
+        hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
+        return prepare(h, dI0, dI1, true, st);
+    });
 }
 
 int of_tvl1_scale_images(const of_tvl1* h, int s, const float** dI0s, const float** dI1s)
@@ -655,14 +663,16 @@ int of_tvl1_scale_images(const of_tvl1* h, int s, const float** dI0s, const floa
 int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int first_scale, int warps,
                            int max_iterations, float* dU, void* stream)
 {
-    if (!h || !dU || warps < 0 || warps > h->prm.warps || max_iterations < 0) return OF_ERR_ARG;
-    if (first_scale < -1 || first_scale >= h->ns) return OF_ERR_ARG;
-    if (!h->prepared) return OF_ERR_STATE;
-    hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
-    if (first_scale < 0) first_scale = h->ns - 1;
-    if (warps == 0) warps = h->prm.warps;
-    if (max_iterations == 0) max_iterations = h->prm.max_iterations;
-    return solve_levels(h, dUinit, first_scale, warps, max_iterations, dU, st);
+    return guarded([&]() -> int {
+        if (!h || !dU || warps < 0 || warps > h->prm.warps || max_iterations < 0) return OF_ERR_ARG;
+        if (first_scale < -1 || first_scale >= h->ns) return OF_ERR_ARG;
+        if (!h->prepared) return OF_ERR_STATE;
+        hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
+        if (first_scale < 0) first_scale = h->ns - 1;
+        if (warps == 0) warps = h->prm.warps;
+        if (max_iterations == 0) max_iterations = h->prm.max_iterations;
+        return solve_levels(h, dUinit, first_scale, warps, max_iterations, dU, st);
+    });
 }
 
 int of_tvl1_lift_flow(of_tvl1* h, int from_scale, const float* dUfrom, int to_scale, float* dUto,
@@ -750,47 +760,53 @@ int of_tvl1_energy(of_tvl1* h, int s, const float* dUs, double* energy, void* st
 
 int of_tvl1_run_u8(of_tvl1* h, const uint8_t* I0, const uint8_t* I1)
 {
-    if (!h || !I0 || !I1) return OF_ERR_ARG;
-    const size_t N0 = (size_t)h->W * h->H;
-    OF_HIP(hipMemcpyAsync(h->in8, I0, N0, hipMemcpyHostToDevice, h->stream));
-    OF_HIP(hipMemcpyAsync(h->in8 + N0, I1, N0, hipMemcpyHostToDevice, h->stream));
-    OF_TRY(solve(h, h->in8, h->in8 + N0, true, h->dflow, h->stream));
-    OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
-    OF_HIP(hipStreamSynchronize(h->stream));
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!h || !I0 || !I1) return OF_ERR_ARG;
+        const size_t N0 = (size_t)h->W * h->H;
+        OF_HIP(hipMemcpyAsync(h->in8, I0, N0, hipMemcpyHostToDevice, h->stream));
+        OF_HIP(hipMemcpyAsync(h->in8 + N0, I1, N0, hipMemcpyHostToDevice, h->stream));
+        OF_TRY(solve(h, h->in8, h->in8 + N0, true, h->dflow, h->stream));
+        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
+        OF_HIP(hipStreamSynchronize(h->stream));
+        return OF_OK;
+    });
 }
 
 int of_tvl1_run_f32(of_tvl1* h, const float* I0, const float* I1)
 {
-    if (!h || !I0 || !I1) return OF_ERR_ARG;
-    const size_t N0 = (size_t)h->W * h->H;
-    OF_HIP(hipMemcpyAsync(h->inf, I0, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-    OF_HIP(hipMemcpyAsync(h->inf + N0, I1, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-    OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
-    OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
-    OF_HIP(hipStreamSynchronize(h->stream));
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!h || !I0 || !I1) return OF_ERR_ARG;
+        const size_t N0 = (size_t)h->W * h->H;
+        OF_HIP(hipMemcpyAsync(h->inf, I0, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+        OF_HIP(hipMemcpyAsync(h->inf + N0, I1, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+        OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
+        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
+        OF_HIP(hipStreamSynchronize(h->stream));
+        return OF_OK;
+    });
 }
 
 int of_tvl1_push_f32(of_tvl1* h, const float* I)
 {
-    if (!h || !I) return OF_ERR_ARG;
-    const size_t N0 = (size_t)h->W * h->H;
-    if (!h->have_prev) {
-        OF_HIP(hipMemcpyAsync(h->prev, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+    return guarded([&]() -> int {
+        if (!h || !I) return OF_ERR_ARG;
+        const size_t N0 = (size_t)h->W * h->H;
+        if (!h->have_prev) {
+            OF_HIP(hipMemcpyAsync(h->prev, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+            OF_HIP(hipStreamSynchronize(h->stream));
+            std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
+            h->have_prev = true;
+            return OF_OK;
+        }
+        // I0 = previous frame, I1 = this frame (src/tvl1/cuda/tvl1.cu:126-130)
+        OF_HIP(hipMemcpyAsync(h->inf, h->prev, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
+        OF_HIP(hipMemcpyAsync(h->inf + N0, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
+        OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
+        OF_HIP(hipMemcpyAsync(h->prev, h->inf + N0, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
+        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
         OF_HIP(hipStreamSynchronize(h->stream));
-        std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
-        h->have_prev = true;
         return OF_OK;
-    }
-    // I0 = previous frame, I1 = this frame (src/tvl1/cuda/tvl1.cu:126-130)
-    OF_HIP(hipMemcpyAsync(h->inf, h->prev, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
-    OF_HIP(hipMemcpyAsync(h->inf + N0, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-    OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
-    OF_HIP(hipMemcpyAsync(h->prev, h->inf + N0, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
-    OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
-    OF_HIP(hipStreamSynchronize(h->stream));
-    return OF_OK;
+    });
 }
 
 const float* of_tvl1_flow(of_tvl1* h) { return h ? h->hflow : nullptr; }
@@ -829,8 +845,10 @@ int of_tvl1_get_stats(const of_tvl1* h, of_tvl1_stats* st)
 int of_tvl1_autotune(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1, int reps, int* chosen,
                      double* ms_table)
 {
-    if (!h || !dI0 || !dI1) return OF_ERR_ARG;
-    return autotune(h, dI0, dI1, reps, chosen, ms_table);
+    return guarded([&]() -> int {
+        if (!h || !dI0 || !dI1) return OF_ERR_ARG;
+        return autotune(h, dI0, dI1, reps, chosen, ms_table);
+    });
 }
 
 int of_tvl1_set_inflight(of_tvl1* h, int launches)
@@ -842,11 +860,13 @@ int of_tvl1_set_inflight(of_tvl1* h, int launches)
 
 int of_tvl1_profile_enable(of_tvl1* h, int enable)
 {
-    if (!h) return OF_ERR_ARG;
-    h->profiling = enable != 0;
-    h->prof.clear();
-    h->timed.clear();
-    return OF_OK;
+    return guarded([&]() -> int {
+        if (!h) return OF_ERR_ARG;
+        h->profiling = enable != 0;
+        h->prof.clear();
+        h->timed.clear();
+        return OF_OK;
+    });
 }
 
 int of_tvl1_profile_get(const of_tvl1* h, of_tvl1_kernel_profile* out, int cap, int* count)
@@ -861,22 +881,24 @@ int of_tvl1_profile_get(const of_tvl1* h, of_tvl1_kernel_profile* out, int cap, 
 int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_launch, int rows_per_wave,
                             int persistent)
 {
-    if (!h) return OF_ERR_ARG;
-    if (min_pixels < 0) {
-        h->rules.clear();
-        return OF_OK;
-    }
-    if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
-    const int p = persistent ? 1 : 0;
-    for (LaunchRule& r : h->rules)
-        if (r.min_pixels == min_pixels) {
-            r.K = iters_per_launch;
-            r.R = rows_per_wave;
-            r.persistent = p;
+    return guarded([&]() -> int {
+        if (!h) return OF_ERR_ARG;
+        if (min_pixels < 0) {
+            h->rules.clear();
             return OF_OK;
         }
-    h->rules.push_back({min_pixels, iters_per_launch, rows_per_wave, p});
-    return OF_OK;
+        if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
+        const int p = persistent ? 1 : 0;
+        for (LaunchRule& r : h->rules)
+            if (r.min_pixels == min_pixels) {
+                r.K = iters_per_launch;
+                r.R = rows_per_wave;
+                r.persistent = p;
+                return OF_OK;
+            }
+        h->rules.push_back({min_pixels, iters_per_launch, rows_per_wave, p});
+        return OF_OK;
+    });
 }
 
 }  // extern "C"
