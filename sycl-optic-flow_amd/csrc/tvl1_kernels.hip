@@ -437,18 +437,33 @@ __global__ __launch_bounds__(256) void k_energy(const float* __restrict__ I0,
     }
 }
 
-__global__ void k_energy_final(const double* __restrict__ part, int nblocks, double lambda,
-                               double* __restrict__ out)
+// One block of 256: each thread sums the partials b = tid, tid+256, ... in
+// order, then a fixed-order tree, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__ part,
+                                                      int nblocks, double lambda,
+                                                      double* __restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double sd[256], st[256];
     double d = 0.0, t = 0.0;
-    for (int b = 0; b < nblocks; b++) {
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
         d += part[2 * b];
         t += part[2 * b + 1];
     }
-    out[0] = lambda * d + t;
-    out[1] = d;
-    out[2] = t;
+    sd[threadIdx.x] = d;
+    st[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+            st[threadIdx.x] += st[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = lambda * sd[0] + st[0];
+        out[1] = sd[0];
+        out[2] = st[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -626,7 +641,7 @@ int launch_energy(const float* I0, const float* I1, const float* u1, const float
     const unsigned blocks = std::min(grid1d((size_t)nx * ny), (unsigned)max_partials);
     hipLaunchKernelGGL(k_energy, dim3(blocks), dim3(256), 0, s, I0, I1, u1, u2, nx, ny, partials);
     OF_LAUNCH_CHECK("k_energy");
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, s, partials, (int)blocks,
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partials, (int)blocks,
                        (double)lambda, out);
     OF_LAUNCH_CHECK("k_energy_final");
     return OF_OK;
