@@ -170,17 +170,25 @@ int of_tvl1_profile_get(const of_tvl1 *h, of_tvl1_kernel_profile *out, int cap, 
 
 /* ------------------------------------------------------------------------
  * Batched TV-L1 (no reference counterpart): many frame pairs of mixed sizes
- * solved concurrently by `streams` host workers, each with its own HIP
- * stream and one solver handle per frame size, taking pairs longest first.
- * Results are identical to solving each pair alone.  The workers' streams
- * do not wait for other streams: the frames must be complete on the device
- * when of_tvl1_batch_run_u8 is called.
+ * solved together.  Results are identical to solving each pair alone.
+ *   OF_BATCH_LOCKSTEP (default): one solver handle per pair (pooled by frame
+ *     size); the pairs are prepared on `streams` streams, then solved scale
+ *     by scale with every launch covering that scale of all pairs; each
+ *     pair stops on its own rule.  Groups of up to 512 pairs.
+ *   OF_BATCH_WORKERS: `streams` host workers, each with its own HIP stream
+ *     and one handle per frame size, solve pairs one by one, longest first.
+ * The batch's streams do not wait for other streams: the frames must be
+ * complete on the device when of_tvl1_batch_run_u8 is called.
  * ---------------------------------------------------------------------- */
 typedef struct of_tvl1_batch of_tvl1_batch;
+enum { OF_BATCH_LOCKSTEP = 0, OF_BATCH_WORKERS = 1 };
 int of_tvl1_batch_create(const of_tvl1_params *params, int streams, of_tvl1_batch **out);
 void of_tvl1_batch_destroy(of_tvl1_batch *b);
-/* Launch shape for scales of at least `min_pixels` pixels in this batch's
- * handles (host-paced launches). */
+/* OF_BATCH_LOCKSTEP or OF_BATCH_WORKERS. */
+int of_tvl1_batch_set_mode(of_tvl1_batch *b, int mode);
+/* Launch shape (iterations per launch, rows per wave) for scales of at
+ * least `min_pixels` pixels: in lockstep mode chosen by the largest pair
+ * at each scale, in worker mode applied to every handle. */
 int of_tvl1_batch_set_shape(of_tvl1_batch *b, long long min_pixels, int iters_per_launch,
                             int rows_per_wave);
 /* Solves pair i: device u8 frames dI0[i], dI1[i] of widths[i] x heights[i],

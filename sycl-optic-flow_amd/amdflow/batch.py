@@ -16,18 +16,31 @@ _lib.of_tvl1_batch_destroy.restype = None
 _lib.of_tvl1_batch_destroy.argtypes = [_vp]
 _lib.of_tvl1_batch_set_shape.restype = C.c_int
 _lib.of_tvl1_batch_set_shape.argtypes = [_vp, C.c_longlong, C.c_int, C.c_int]
+_lib.of_tvl1_batch_set_mode.restype = C.c_int
+_lib.of_tvl1_batch_set_mode.argtypes = [_vp, C.c_int]
 _lib.of_tvl1_batch_run_u8.restype = C.c_int
 _lib.of_tvl1_batch_run_u8.argtypes = [_vp, C.c_int, _ip, _ip, C.POINTER(_vp), C.POINTER(_vp),
                                       C.POINTER(_vp), _ip]
 
 
+MODES = {"lockstep": 0, "workers": 1}   # OF_BATCH_LOCKSTEP, OF_BATCH_WORKERS
+
+
 class TVL1Batch:
+    """mode "lockstep" (default): every launch covers one scale of all pairs;
+    "workers": `streams` host threads solve pairs one by one."""
+
     def __init__(self, streams: int = 4, tau=0.25, lambda_=0.15, theta=0.3, nscales=100,
-                 zfactor=0.5, warps=5, epsilon=0.01, max_iterations=300):
+                 zfactor=0.5, warps=5, epsilon=0.01, max_iterations=300, mode: str = "lockstep"):
         p = TVL1Params(tau, lambda_, theta, nscales, zfactor, warps, epsilon, max_iterations)
         h = _vp()
         _check(_lib.of_tvl1_batch_create(C.byref(p), int(streams), C.byref(h)), "batch_create")
         self._h = h
+        self.set_mode(mode)
+
+    def set_mode(self, mode: str) -> None:
+        _check(_lib.of_tvl1_batch_set_mode(self._h, MODES[mode]), "batch_set_mode")
+        self.mode = mode
 
     def set_shape(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int) -> None:
         _check(_lib.of_tvl1_batch_set_shape(self._h, int(min_pixels), int(iters_per_launch),

@@ -28,6 +28,7 @@ SOURCES = [
     "tvl1.cpp",
     "debug.cpp",
     "batch.cpp",
+    "batch_lockstep.hip",
     "hs.hip",
     "lk.hip",
     "cnn.hip",

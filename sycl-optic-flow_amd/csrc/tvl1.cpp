@@ -18,18 +18,12 @@
 #include "amdflow.h"
 #include "common.hpp"
 #include "tvl1_kernels.hpp"
+#include "tvl1_handle.hpp"
 #include "tvl1_launch.hpp"
 
 using namespace amdflow;
 
 namespace {
-
-struct LaunchRule {
-    long long min_pixels;
-    int K;
-    int R;
-    int persistent;   // 1: whole warping in one cooperative launch
-};
 
 // Built-in launch shapes per scale size (pixels): large scales are bound by
 // memory traffic and use short temporal blocking with host-paced launches;
@@ -41,67 +35,9 @@ const LaunchRule kDefaultRules[] = {
     {0, 4, 4, 1},
 };
 
-constexpr int kInFlight = 3;   // iteration launches kept queued ahead of the device
 constexpr int kEnergyBlocks = 1024;
 
 }  // namespace
-
-struct of_tvl1 {
-    of_tvl1_params prm{};
-    int W = 0, H = 0, ns = 0;
-    int nx[kMaxScales]{}, ny[kMaxScales]{};
-    size_t off[kMaxScales]{};
-    size_t total = 0;
-
-    hipStream_t stream = nullptr;
-    float* I0p = nullptr;
-    float* I1p = nullptr;
-    float* ubase = nullptr;     // per scale s at 4*off[s]: [buf0 u1|u2][buf1 u1|u2]
-    float* pbase = nullptr;     // [buf0 p11|p12|p21|p22][buf1 ...], buffer stride 4*N0
-    float* wsrc = nullptr;      // 4*N0: (I1, I1x, I1y, 0) per pixel of the active scale
-    float* wx = nullptr;
-    float* wy = nullptr;
-    float* rhoc = nullptr;
-    float* A0 = nullptr;
-    float* A1 = nullptr;
-    uint8_t* in8 = nullptr;     // 2*N0 bytes of upload staging
-    float* inf = nullptr;       // 2*N0 floats of upload staging
-    float* prev = nullptr;      // previous frame of the streaming API
-    bool have_prev = false;
-    float* dflow = nullptr;     // 2*N0
-    float* hflow = nullptr;     // pinned host copy returned by of_tvl1_flow
-    TvCtl* ctl = nullptr;
-    double* partials = nullptr;
-    double* energy = nullptr;   // 2*kEnergyBlocks partials, then (E, data, TV)
-    bool prepared = false;      // pyramids hold the last prepared pair
-    bool solved = false;        // per-scale flows hold the last solve
-    int run_warps = 0, run_max = 0;
-    int* d_iters = nullptr;
-    unsigned long long* hstatus = nullptr;   // host-mapped
-    unsigned epoch = 0;
-    std::vector<int> h_iters;
-    GaussTaps g_pre{}, g_zoom{};
-    std::vector<LaunchRule> rules;
-    long long launches_issued = 0;
-    int inflight = kInFlight;
-    double host_launch_s = 0.0, host_wait_s = 0.0;
-
-    // per-launch event timing (of_tvl1_profile_enable)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;
-    struct Timed {
-        int ev;          // index of the start event; end event is ev+1
-        int K, R, persistent;
-    };
-    std::vector<Timed> timed;
-    std::vector<of_tvl1_kernel_profile> prof;
-    int scale_K[kMaxScales]{}, scale_R[kMaxScales]{}, scale_P[kMaxScales]{};
-
-    // per-scale event timing (autotuner)
-    bool scale_timing = false;
-    std::vector<hipEvent_t> scale_ev;
-    double scale_ms[kMaxScales]{};
-};
 
 namespace {
 
@@ -307,10 +243,9 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     return OF_OK;
 }
 
-// Enqueues the first phase of a solve: frames to float, joint
-// normalisation, pre-smoothing and both pyramids.  `is_u8` selects the
-// element type of src0/src1.
-int prepare(of_tvl1* h, const void* src0, const void* src1, bool is_u8, hipStream_t st)
+}  // namespace
+
+int amdflow::tvl1_prepare(of_tvl1* h, const void* src0, const void* src1, bool is_u8, hipStream_t st)
 {
     const of_tvl1_params& P = h->prm;
     const size_t N0 = (size_t)h->W * h->H;
@@ -341,6 +276,8 @@ int prepare(of_tvl1* h, const void* src0, const void* src1, bool is_u8, hipStrea
     h->solved = false;
     return OF_OK;
 }
+
+namespace {
 
 // Second phase: the coarse-to-fine solve from the prepared pyramids, with
 // `init` (planar [u1|u2] at the coarsest scale, device) as the initial flow,
@@ -411,7 +348,7 @@ int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_it
 // One full multiscale solve with the handle's parameters and zero initial flow.
 int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst, hipStream_t st)
 {
-    OF_TRY(prepare(h, src0, src1, is_u8, st));
+    OF_TRY(tvl1_prepare(h, src0, src1, is_u8, st));
     return solve_levels(h, nullptr, h->ns - 1, h->prm.warps, h->prm.max_iterations, dst, st);
 }
 
@@ -647,7 +584,7 @@ int of_tvl1_prepare_device_u8(of_tvl1* h, const uint8_t* dI0, const uint8_t* dI1
     return guarded([&]() -> int {
         if (!h || !dI0 || !dI1) return OF_ERR_ARG;
         hipStream_t st = (hipStream_t)stream;   // NULL: the HIP null stream
-        return prepare(h, dI0, dI1, true, st);
+        return tvl1_prepare(h, dI0, dI1, true, st);
     });
 }
 

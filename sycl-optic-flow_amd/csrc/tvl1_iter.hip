@@ -10,7 +10,10 @@
 //                        data (small, launch-latency-bound scales).
 // Arithmetic follows the reference operation by operation; the library is
 // built with -ffp-contract=off.
+#include <algorithm>
+
 #include "device_math.hpp"
+#include "lockstep.hpp"
 #include "tvl1_kernels.hpp"
 
 namespace amdflow {
@@ -481,6 +484,176 @@ __global__ __launch_bounds__(256) void k_tvl1_persist(IterArgs a, unsigned tiles
 }
 
 // ---------------------------------------------------------------------------
+// lockstep driver: one scale of many pairs per launch
+// ---------------------------------------------------------------------------
+
+// Inclusive prefix sums of v[0..n), n <= kLsMaxPairs, by one 256-thread
+// workgroup (Hillis-Steele over LDS).
+__device__ __forceinline__ void ls_block_scan(unsigned* v, int n)
+{
+    for (int off = 1; off < n; off <<= 1) {
+        unsigned t[kLsMaxPairs / 256];
+#pragma unroll
+        for (int k = 0; k < kLsMaxPairs / 256; k++) {
+            const int e = (int)threadIdx.x + 256 * k;
+            t[k] = (e < n && e >= off) ? v[e - off] : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kLsMaxPairs / 256; k++) {
+            const int e = (int)threadIdx.x + 256 * k;
+            if (e < n) v[e] += t[k];
+        }
+        __syncthreads();
+    }
+}
+
+// Iterations of one pair's tile `bid` (kk <= K of them) and, in the
+// workgroup that finishes the pair's last tile of this launch, the pair's
+// stopping decision: the same partial sums, order and rule as k_tvl1_iter.
+template <int K, int R>
+__device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsigned bid,
+                                        TileShared& sh, double (*sh_err)[K], double* sh_tot,
+                                        double* sh4, int* sh_last)
+{
+    // the table entry is read in place: a private copy would be indexed by
+    // `cur` at run time and live in scratch memory
+    const IterArgs& a = P->a;
+    TvCtl* ctl = a.ctl;
+    const int cur = ctl->cur[a.level];
+    const int redo = ctl->redo;
+    const int n0 = ctl->n;
+    int kk = redo ? redo : a.max_iter - n0;
+    if (kk > K) kk = K;
+    const unsigned nblocks = P->ntiles;
+    double acc[K];
+#pragma unroll
+    for (int t = 0; t < K; t++) acc[t] = 0.0;
+    if (kk > 0) tile_iterate<K, R, false>(a, (int)(bid % P->gx), (int)(bid / P->gx), cur, kk, acc, sh);
+    const double part = block_sum<K>(acc, sh_err, threadIdx.x);
+    if ((int)threadIdx.x < kk)
+        __hip_atomic_store(a.partials + (size_t)threadIdx.x * nblocks + bid, part, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (threadIdx.x == 0) *sh_last = atomicAdd(&ctl->counter, 1u) == nblocks - 1;
+    __syncthreads();
+    if (*sh_last) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int t = 0; t < kk; t++) {
+            const double tot = grid_sum(a.partials + (size_t)t * nblocks, nblocks, sh4);
+            if (threadIdx.x == 0) sh_tot[t] = tot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int n = n0, stop = 0, nredo = 0, flip = 0;
+            if (kk <= 0) {   // nothing left to do: only reached if the list was stale
+                stop = 1;
+            } else if (redo) {
+                n = n0 + redo;
+                stop = 1;
+                flip = 1;
+            } else {
+                const int keep = stop_after(sh_tot, kk, n0, a);
+                if (keep == kk || keep == 0) {
+                    n = n0 + kk;
+                    flip = 1;
+                    stop = keep != 0;
+                } else {
+                    nredo = keep;
+                }
+            }
+            ctl->n = n;
+            ctl->redo = nredo;
+            if (flip) ctl->cur[a.level] = cur ^ 1;
+            if (stop)
+                ph.iters[((size_t)P->pair_id * kMaxScales + a.level) * ph.warps + ph.warp] = n;
+            ctl->counter = 0;
+            ctl->launches = ctl->launches + 1;
+            // read in this launch by the workgroup that rebuilds the work list
+            __hip_atomic_store(&ctl->stop, stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+}
+
+template <int K, int R>
+__global__ __launch_bounds__(256) void k_ls_iter(LsPhase ph, const LsPair* __restrict__ pairs,
+                                                 LsCtl* mc, unsigned long long* host_status,
+                                                 unsigned epoch)
+{
+    // `pairs` is ph.pairs as a restrict kernel argument: the compiler may
+    // then read the (wave-uniform) table entries with scalar loads
+    __shared__ TileShared sh;
+    __shared__ double sh_err[4][K];
+    __shared__ double sh_tot[K];
+    __shared__ double sh4[4];
+    __shared__ int sh_last;
+    __shared__ unsigned s_pair[kLsMaxPairs], s_base[kLsMaxPairs];
+
+    const unsigned nlist = mc->nlist;
+    const unsigned ntot = mc->ntiles;
+    for (unsigned k = threadIdx.x; k < nlist; k += 256) {
+        s_pair[k] = mc->list_pair[k];
+        s_base[k] = mc->list_base[k];
+    }
+    __syncthreads();
+    for (unsigned t = blockIdx.x; t < ntot; t += gridDim.x) {
+        int lo = 0, hi = (int)nlist - 1;   // last listed pair whose first entry is <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_base[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        // the owner is the same in every lane: keep it (and the table entry's
+        // fields) in scalar registers
+        const unsigned pk = __builtin_amdgcn_readfirstlane(s_pair[lo]);
+        const unsigned bid = __builtin_amdgcn_readfirstlane(t - s_base[lo]);
+        ls_tile<K, R>(ph, pairs + pk, bid, sh, sh_err, sh_tot, sh4, &sh_last);
+    }
+
+    // launch completion: the last workgroup rebuilds the work list
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (threadIdx.x == 0) sh_last = atomicAdd(&mc->counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!sh_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int np = ph.npairs;
+    for (int k = threadIdx.x; k < np; k += 256) {
+        const int stop =
+            __hip_atomic_load(&ph.pairs[k].a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_pair[k] = stop ? 0u : 1u;
+        s_base[k] = stop ? 0u : ph.pairs[k].ntiles;
+    }
+    __syncthreads();
+    ls_block_scan(s_pair, np);
+    ls_block_scan(s_base, np);
+    for (int k = threadIdx.x; k < np; k += 256) {
+        const unsigned before = k ? s_pair[k - 1] : 0u;
+        if (s_pair[k] != before) {   // pair k still iterates
+            mc->list_pair[before] = (unsigned)k;
+            mc->list_base[before] = k ? s_base[k - 1] : 0u;
+        }
+    }
+    if (threadIdx.x == 0) {
+        const unsigned left = np ? s_pair[np - 1] : 0u;
+        mc->nlist = left;
+        mc->ntiles = np ? s_base[np - 1] : 0u;
+        mc->counter = 0;
+        const unsigned done = mc->launches + 1;
+        mc->launches = done;
+        const unsigned long long word = ((unsigned long long)epoch << 32) |
+                                        ((unsigned long long)(left == 0) << 31) |
+                                        (unsigned long long)(done & 0x7FFFFFFFu);
+        if (host_status)
+            __hip_atomic_store(host_status, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
@@ -568,6 +741,59 @@ int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s)
     OF_PCASE(1, 8) OF_PCASE(2, 8) OF_PCASE(4, 8) OF_PCASE(8, 8)
 #undef OF_PCASE
     return OF_ERR_UNSUPPORTED;
+}
+
+template <int K, int R>
+static hipError_t launch_ls_kr(const LsPhase& ph, LsCtl* mc, unsigned grid,
+                               unsigned long long* host_status, unsigned epoch, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_ls_iter<K, R>), dim3(grid), dim3(256), 0, s, ph, ph.pairs, mc, host_status,
+                       epoch);
+    return hipGetLastError();
+}
+
+int launch_ls_iter(int K, int R, const LsPhase& ph, LsCtl* mc, unsigned grid,
+                   unsigned long long* host_status, unsigned epoch, hipStream_t s)
+{
+    if (grid == 0 || ph.npairs < 1 || ph.npairs > kLsMaxPairs) return OF_ERR_ARG;
+    hipError_t e = hipErrorInvalidValue;
+#define OF_CASE(k, r) \
+    if (K == k && R == r) e = launch_ls_kr<k, r>(ph, mc, grid, host_status, epoch, s);
+    OF_CASE(1, 2) OF_CASE(2, 2)
+    OF_CASE(1, 4) OF_CASE(2, 4) OF_CASE(4, 4)
+    OF_CASE(1, 8) OF_CASE(2, 8) OF_CASE(4, 8) OF_CASE(8, 8)
+#undef OF_CASE
+    if (e != hipSuccess) {
+        record_hip_error(e, "k_ls_iter launch", __FILE__, __LINE__);
+        return OF_ERR_HIP;
+    }
+    return OF_OK;
+}
+
+template <int K, int R>
+static int ls_resident_kr(unsigned* blocks)
+{
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ls_iter<K, R>, 256, 0) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return OF_ERR_HIP;
+    }
+    *blocks = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus);
+    return OF_OK;
+}
+
+int ls_iter_resident(int K, int R, unsigned* blocks)
+{
+#define OF_CASE(k, r) \
+    if (K == k && R == r) return ls_resident_kr<k, r>(blocks);
+    OF_CASE(1, 2) OF_CASE(2, 2)
+    OF_CASE(1, 4) OF_CASE(2, 4) OF_CASE(4, 4)
+    OF_CASE(1, 8) OF_CASE(2, 8) OF_CASE(4, 8) OF_CASE(8, 8)
+#undef OF_CASE
+    return OF_ERR_ARG;
 }
 
 }  // namespace amdflow

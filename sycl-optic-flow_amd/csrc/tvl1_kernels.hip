@@ -5,6 +5,7 @@
 // the library is built with -ffp-contract=off so no multiply-add is fused.
 #include "device_math.hpp"
 #include "tvl1_kernels.hpp"
+#include "tvl1_pixel.hpp"
 #include "tvl1_launch.hpp"
 
 namespace amdflow {
@@ -250,14 +251,7 @@ __global__ __launch_bounds__(256) void k_zoom_in_flow(const float* __restrict__ 
     const int i1 = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j1 >= nxx || i1 >= nyy) return;
     const float* uc = ctl->cur[coarse_level] ? uc1 : uc0;
-    const size_t nc = (size_t)nx * ny, nf = (size_t)nxx * nyy;
-    const float i2 = (float)i1 / factory;
-    const float j2 = (float)j1 / factorx;
-    const BicubicTaps t = bicubic_taps(j2, i2, nx, ny);
-    const float a = bicubic_sample(uc, nx, t);
-    const float b = bicubic_sample(uc + nc, nx, t);
-    uf[(size_t)i1 * nxx + j1] = invz * a;
-    uf[nf + (size_t)i1 * nxx + j1] = invz * b;
+    zoom_in_flow_px(uc, uf, nx, ny, nxx, nyy, factorx, factory, invz, i1, j1);
 }
 
 // zoom_in of two separate planes times `scale` (used by the pyramidal LK).
@@ -306,12 +300,7 @@ __global__ __launch_bounds__(256) void k_centered_gradient_packed(const float* _
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j >= nx || i >= ny) return;
-    const int jl = j > 0 ? j - 1 : 0, jr = j < nx - 1 ? j + 1 : nx - 1;
-    const int iu = i > 0 ? i - 1 : 0, id = i < ny - 1 ? i + 1 : ny - 1;
-    const float ddx = in[(size_t)i * nx + jr] - in[(size_t)i * nx + jl];
-    const float ddy = in[(size_t)id * nx + j] - in[(size_t)iu * nx + j];
-    out[(size_t)i * nx + j] =
-        make_float4(in[(size_t)i * nx + j], (float)(0.5 * (double)ddx), (float)(0.5 * (double)ddy), 0.0f);
+    gradient_packed_px(in, out, nx, ny, i, j);
 }
 
 __global__ __launch_bounds__(256) void k_pack3(const float* __restrict__ a, const float* __restrict__ b,
@@ -361,18 +350,7 @@ __global__ __launch_bounds__(256) void k_warp_setup(const float* __restrict__ I0
         ctl->wlaunches = 0;
     }
     if (j >= nx || i >= ny) return;
-    const size_t plane = (size_t)nx * ny;
-    const float* ub = ctl->cur[level] ? ub1 : ub0;
-    const size_t p = (size_t)i * nx + j;
-    const float u1 = ub[p], u2 = ub[plane + p];
-    const float uu = (float)j + u1;
-    const float vv = (float)i + u2;
-    const BicubicTaps t = bicubic_taps(uu, vv, nx, ny);
-    float w = 0.0f, gx = 0.0f, gy = 0.0f;
-    if (!t.out) bicubic_sample3(src, nx, t, w, gx, gy);
-    wx[p] = gx;
-    wy[p] = gy;
-    rhoc[p] = ((w - gx * u1) - gy * u2) - I0[p];
+    warp_setup_px(I0, src, ctl->cur[level] ? ub1 : ub0, wx, wy, rhoc, nx, ny, i, j);
 }
 
 // Copies the finest scale's current flow into the caller's planar buffer.

@@ -1,32 +1,91 @@
 """The batched TV-L1 solver gives, for every pair of a mixed-size batch, the
-same flow as the single-pair solver (and hence the oracle)."""
+same flow and iteration counts as the single-pair solver (and hence the
+oracle), in both batch modes."""
 import numpy as np
 import pytest
 
 import oracle as O
-from amdflow import synth
+from amdflow import TV_L1, synth
 
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_matches_single_pair_solves():
+def _frames(sizes, cap, seed0):
     import torch
-    from amdflow.batch import TVL1Batch
-
-    sizes = [(160, 120), (200, 150), (320, 240), (97, 61), (160, 120), (256, 192), (320, 240)]
-    frames = []
+    frames, d0, d1 = [], [], []
     for k, (w, h) in enumerate(sizes):
-        I0, I1, _, _ = synth.make_pair(w, h, 3.0, 100 + k, bumps=True)
-        frames.append((synth.quantize(I0), synth.quantize(I1)))
-    d0 = [torch.from_numpy(a.reshape(-1)).cuda() for a, _ in frames]
-    d1 = [torch.from_numpy(b.reshape(-1)).cuda() for _, b in frames]
-    dU = [torch.empty(2 * w * h, dtype=torch.float32, device="cuda") for (w, h) in sizes]
+        I0, I1, _, _ = synth.make_pair(w, h, cap, seed0 + k, bumps=True)
+        q0, q1 = synth.quantize(I0), synth.quantize(I1)
+        frames.append((q0, q1))
+        d0.append(torch.from_numpy(q0.reshape(-1)).cuda())
+        d1.append(torch.from_numpy(q1.reshape(-1)).cuda())
+    return frames, d0, d1
+
+
+def _run(b, sizes, d0, d1):
+    import torch
+    dU = [torch.full((2 * w * h,), float("nan"), dtype=torch.float32, device="cuda") for (w, h) in sizes]
     torch.cuda.synchronize()
-    with TVL1Batch(streams=4) as b:
-        it = b.run_device_u8([w for w, _ in sizes], [h for _, h in sizes],
-                             [t.data_ptr() for t in d0], [t.data_ptr() for t in d1],
-                             [t.data_ptr() for t in dU])
+    it = b.run_device_u8([w for w, _ in sizes], [h for _, h in sizes],
+                         [t.data_ptr() for t in d0], [t.data_ptr() for t in d1],
+                         [t.data_ptr() for t in dU])
+    return it, [t.cpu().numpy() for t in dU]
+
+
+@pytest.mark.parametrize("mode", ["lockstep", "workers"])
+def test_batch_matches_oracle(mode):
+    from amdflow.batch import TVL1Batch
+    # mixed sizes with 3, 4 and 5 scales (default nscales rule), two of one size
+    sizes = [(160, 120), (200, 150), (320, 240), (97, 61), (160, 120), (256, 192), (320, 240)]
+    frames, d0, d1 = _frames(sizes, 3.0, 100)
+    with TVL1Batch(streams=4, mode=mode) as b:
+        it, flows = _run(b, sizes, d0, d1)
     for k, (q0, q1) in enumerate(frames):
         uo, ito = O.tvl1(q0, q1)
-        assert it[k] == int(ito.sum())
-        assert np.array_equal(dU[k].cpu().numpy(), uo), k
+        assert it[k] == int(ito.sum()), (mode, k)
+        assert np.array_equal(flows[k], uo), (mode, k)
+
+
+def test_lockstep_matches_single_solves_1080p_mix():
+    """A 1920x1080 pair among smaller ones (5 scales each), solved twice by
+    one batch object (the second time a subset in another order, reusing
+    its pooled handles), against single-pair GPU solves."""
+    import torch
+    from amdflow.batch import TVL1Batch
+    sizes = [(1920, 1080), (320, 240), (1024, 436), (640, 480), (320, 240), (1280, 720)]
+    frames, d0, d1 = _frames(sizes, 20.0, 300)
+    ref_flows, ref_its = [], []
+    for k, (w, h) in enumerate(sizes):
+        tv = TV_L1(w, h, nscales=5)
+        out = torch.empty(2 * w * h, dtype=torch.float32, device="cuda")
+        tv.run_device_u8(d0[k].data_ptr(), d1[k].data_ptr(), out.data_ptr())
+        ref_flows.append(out.cpu().numpy())
+        ref_its.append(int(tv.iterations().sum()))
+        tv.close()
+    with TVL1Batch(streams=4, nscales=5) as b:
+        it, flows = _run(b, sizes, d0, d1)
+        for k in range(len(sizes)):
+            assert it[k] == ref_its[k], k
+            assert np.array_equal(flows[k], ref_flows[k]), k
+        order = [4, 0, 2]
+        it2, flows2 = _run(b, [sizes[k] for k in order], [d0[k] for k in order],
+                           [d1[k] for k in order])
+        for j, k in enumerate(order):
+            assert it2[j] == ref_its[k], k
+            assert np.array_equal(flows2[j], ref_flows[k]), k
+
+
+@pytest.mark.parametrize("K,R", [(1, 2), (2, 4), (4, 8), (8, 8)])
+def test_lockstep_launch_shapes_agree(K, R):
+    """Every launch shape gives the oracle's flow (the shape only changes the
+    tiling, not the arithmetic)."""
+    from amdflow.batch import TVL1Batch
+    sizes = [(160, 120), (97, 61), (200, 150)]
+    frames, d0, d1 = _frames(sizes, 3.0, 500)
+    with TVL1Batch(streams=2) as b:
+        b.set_shape(0, K, R)
+        it, flows = _run(b, sizes, d0, d1)
+    for k, (q0, q1) in enumerate(frames):
+        uo, ito = O.tvl1(q0, q1)
+        assert it[k] == int(ito.sum()), k
+        assert np.array_equal(flows[k], uo), k
