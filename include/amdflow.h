@@ -172,9 +172,9 @@ int of_tvl1_profile_get(const of_tvl1 *h, of_tvl1_kernel_profile *out, int cap, 
  * Batched TV-L1 (no reference counterpart): many frame pairs of mixed sizes
  * solved together.  Results are identical to solving each pair alone.
  *   OF_BATCH_LOCKSTEP (default): one solver handle per pair (pooled by frame
- *     size); the pairs are prepared on `streams` streams, then solved scale
- *     by scale with every launch covering that scale of all pairs; each
- *     pair stops on its own rule.  Groups of up to 512 pairs.
+ *     size); every launch covers all pairs of a lane: the preparation step
+ *     by step, then the solve scale by scale and warping by warping; each
+ *     pair stops on its own rule.  Up to 512 pairs per lane at a time.
  *   OF_BATCH_WORKERS: `streams` host workers, each with its own HIP stream
  *     and one handle per frame size, solve pairs one by one, longest first.
  * The batch's streams do not wait for other streams: the frames must be
@@ -184,7 +184,9 @@ typedef struct of_tvl1_batch of_tvl1_batch;
 enum { OF_BATCH_LOCKSTEP = 0, OF_BATCH_WORKERS = 1 };
 int of_tvl1_batch_create(const of_tvl1_params *params, int streams, of_tvl1_batch **out);
 void of_tvl1_batch_destroy(of_tvl1_batch *b);
-/* OF_BATCH_LOCKSTEP or OF_BATCH_WORKERS. */
+/* OF_BATCH_LOCKSTEP or OF_BATCH_WORKERS.  In lockstep mode `streams` is the
+ * number of lanes: groups of pairs (balanced by pixel count) solved
+ * concurrently, each on its own stream, all driven by the calling thread. */
 int of_tvl1_batch_set_mode(of_tvl1_batch *b, int mode);
 /* Launch shape (iterations per launch, rows per wave) for scales of at
  * least `min_pixels` pixels: in lockstep mode chosen by the largest pair

@@ -27,8 +27,9 @@ MODES = {"lockstep": 0, "workers": 1}   # OF_BATCH_LOCKSTEP, OF_BATCH_WORKERS
 
 
 class TVL1Batch:
-    """mode "lockstep" (default): every launch covers one scale of all pairs;
-    "workers": `streams` host threads solve pairs one by one."""
+    """mode "lockstep" (default): every launch covers one scale of all pairs
+    of a lane, `streams` lanes run concurrently; "workers": `streams` host
+    threads solve pairs one by one."""
 
     def __init__(self, streams: int = 4, tau=0.25, lambda_=0.15, theta=0.3, nscales=100,
                  zfactor=0.5, warps=5, epsilon=0.01, max_iterations=300, mode: str = "lockstep"):

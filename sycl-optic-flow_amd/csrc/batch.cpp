@@ -4,9 +4,10 @@
 // modes, both with results identical to solving each pair alone:
 //
 //   lockstep (default): every pair gets a TV-L1 handle of its own (pooled
-//     by frame size).  The pairs are prepared (normalisation, pyramids) on
-//     the batch's streams, then solved scale by scale in lockstep on one
-//     stream: each launch covers the same scale and warping of all pairs.
+//     by frame size).  All work runs on one stream, and every launch covers
+//     all pairs: the preparation (frames to float, normalisation,
+//     pre-smoothing, pyramids) step by step, then the solve scale by scale
+//     and warping by warping.
 //     Iteration launches walk a device work list of (pair, tile) entries;
 //     each pair stops on its own rule and drops out of the list, and the
 //     host only paces launches until every pair of the warping has
@@ -35,6 +36,41 @@
 #include "tvl1_handle.hpp"
 #include "tvl1_launch.hpp"
 
+// One lockstep lane: a group of pairs solved on its own stream, with its
+// own device control block and tables.  The calling thread advances all
+// lanes in turn without blocking, so the launch-latency-bound coarse scales
+// of one lane overlap the bandwidth-bound fine scales of another.
+struct LsLane {
+    hipStream_t stream = nullptr;             // one of the batch's streams
+    amdflow::LsCtl* mc = nullptr;             // device control of the lane
+    unsigned long long* hstatus = nullptr;    // host-mapped progress word
+    unsigned epoch = 0;
+    amdflow::LsPrep* h_preps = nullptr;       // pinned tables, uploaded once per group
+    amdflow::LsPrep* d_preps = nullptr;
+    size_t preps_cap = 0;
+    amdflow::LsPair* h_pairs = nullptr;
+    amdflow::LsPair* d_pairs = nullptr;
+    size_t pairs_cap = 0;
+    unsigned* h_bases = nullptr;
+    unsigned* d_bases = nullptr;
+    size_t bases_cap = 0;
+    int* h_iters = nullptr;
+    int* d_iters = nullptr;
+    size_t iters_cap = 0;
+    std::map<std::pair<int, int>, unsigned> resident;   // (K, R) -> co-resident workgroups
+
+    // the group in flight (see ls_begin / ls_step)
+    std::vector<int> members;                 // batch indices of its pairs
+    std::vector<amdflow::LsPhase> phases;     // per scale
+    std::vector<int> shapeK, shapeR;
+    size_t niters = 0;
+    int s = -1, w = 0;                        // running scale and warping
+    bool done = true;
+    long long issued = 0, completed = 0, max_launches = 0;
+    unsigned grid = 0, run_epoch = 0, spins = 0;
+    std::chrono::steady_clock::time_point t0;
+};
+
 struct of_tvl1_batch {
     of_tvl1_params prm{};
     int streams = 1;
@@ -45,23 +81,9 @@ struct of_tvl1_batch {
     // launch rules applied to every handle: (min_pixels, K, R)
     std::vector<std::tuple<long long, int, int>> rules;
     int mode = OF_BATCH_LOCKSTEP;
-
     // lockstep mode
     std::map<std::pair<int, int>, std::vector<of_tvl1*>> pool;   // handles by frame size
-    std::vector<hipEvent_t> wevents;          // one per stream: joins the preparations
-    amdflow::LsCtl* mc = nullptr;             // device control of the group
-    unsigned long long* hstatus = nullptr;    // host-mapped progress word
-    unsigned epoch = 0;
-    amdflow::LsPair* h_pairs = nullptr;       // pinned tables, uploaded once per group
-    amdflow::LsPair* d_pairs = nullptr;
-    size_t pairs_cap = 0;
-    unsigned* h_bases = nullptr;
-    unsigned* d_bases = nullptr;
-    size_t bases_cap = 0;
-    int* h_iters = nullptr;
-    int* d_iters = nullptr;
-    size_t iters_cap = 0;
-    std::map<std::pair<int, int>, unsigned> resident;   // (K, R) -> co-resident workgroups
+    std::vector<LsLane> lanes;
 };
 
 namespace {
@@ -168,23 +190,19 @@ void ls_release(of_tvl1_batch* b)
     for (auto& kv : b->pool)
         for (of_tvl1* h : kv.second) of_tvl1_destroy(h);
     b->pool.clear();
-    for (hipEvent_t e : b->wevents)
-        if (e) (void)hipEventDestroy(e);
-    b->wevents.clear();
-    amdflow::dev_free(b->mc);
-    amdflow::dev_free(b->d_pairs);
-    amdflow::dev_free(b->d_bases);
-    amdflow::dev_free(b->d_iters);
-    if (b->hstatus) (void)hipHostFree(b->hstatus);
-    if (b->h_pairs) (void)hipHostFree(b->h_pairs);
-    if (b->h_bases) (void)hipHostFree(b->h_bases);
-    if (b->h_iters) (void)hipHostFree(b->h_iters);
-    b->mc = nullptr;
-    b->d_pairs = b->h_pairs = nullptr;
-    b->d_bases = b->h_bases = nullptr;
-    b->d_iters = b->h_iters = nullptr;
-    b->hstatus = nullptr;
-    b->pairs_cap = b->bases_cap = b->iters_cap = 0;
+    for (LsLane& L : b->lanes) {
+        amdflow::dev_free(L.mc);
+        amdflow::dev_free(L.d_preps);
+        amdflow::dev_free(L.d_pairs);
+        amdflow::dev_free(L.d_bases);
+        amdflow::dev_free(L.d_iters);
+        if (L.hstatus) (void)hipHostFree(L.hstatus);
+        if (L.h_preps) (void)hipHostFree(L.h_preps);
+        if (L.h_pairs) (void)hipHostFree(L.h_pairs);
+        if (L.h_bases) (void)hipHostFree(L.h_bases);
+        if (L.h_iters) (void)hipHostFree(L.h_iters);
+    }
+    b->lanes.clear();
 }
 
 // Pinned host table and its device twin of at least n elements.  Only
@@ -209,22 +227,22 @@ int ls_reserve(T** host, T** dev, size_t* cap, size_t n)
     return OF_OK;
 }
 
-int ls_setup(of_tvl1_batch* b)
+int ls_setup(LsLane& L)
 {
-    if (b->mc) return OF_OK;
-    OF_TRY(amdflow::dev_alloc(&b->mc, 1));
-    OF_HIP(hipMemset(b->mc, 0, sizeof(amdflow::LsCtl)));
-    if (hipHostMalloc((void**)&b->hstatus, sizeof(unsigned long long),
+    if (L.mc) return OF_OK;
+    OF_TRY(amdflow::dev_alloc(&L.mc, 1));
+    OF_HIP(hipMemset(L.mc, 0, sizeof(amdflow::LsCtl)));
+    if (hipHostMalloc((void**)&L.hstatus, sizeof(unsigned long long),
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-        b->hstatus = nullptr;
+        L.hstatus = nullptr;
         (void)hipGetLastError();
         return OF_ERR_ALLOC;
     }
-    *b->hstatus = 0ull;
-    b->wevents.assign(b->streams, nullptr);
-    for (hipEvent_t& e : b->wevents) OF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *L.hstatus = 0ull;
     return OF_OK;
 }
+
+constexpr size_t kLsMaxPairsPerGroup = amdflow::kLsMaxPairs;
 
 // Launch shape of scale s: the batch's rule for the largest participating
 // scale size, else (2 iterations, 8 rows) from 200k pixels and (4, 4) below.
@@ -240,120 +258,69 @@ void ls_shape(const of_tvl1_batch* b, long long npx, int* K, int* R)
         }
 }
 
-// Iteration launches of one warping of all pairs of `ph`, kept kInFlight
-// ahead of the device, until the device reports that every pair stopped.
-int ls_run_warping(of_tvl1_batch* b, const amdflow::LsPhase& ph, int K, int R, hipStream_t st)
+// Starts the iteration launches of warping L.w of scale L.s.
+int ls_start_warping(const of_tvl1_batch* b, LsLane& L)
 {
     using namespace amdflow;
+    const int K = L.shapeK[L.s], R = L.shapeR[L.s];
     auto key = std::make_pair(K, R);
-    auto it = b->resident.find(key);
-    if (it == b->resident.end()) {
+    auto it = L.resident.find(key);
+    if (it == L.resident.end()) {
         unsigned blocks = 0;
         OF_TRY(ls_iter_resident(K, R, &blocks));
-        it = b->resident.emplace(key, blocks).first;
+        it = L.resident.emplace(key, blocks).first;
     }
-    const unsigned grid = std::max(1u, std::min(ph.ntiles, it->second));
-    const unsigned epoch = ++b->epoch;
-    const long long max_launches = (b->prm.max_iterations + K - 1) / K + 1;
-    long long issued = 0, completed = 0;
-    bool stopped = false;
-    unsigned spins = 0;
-    const auto t0 = std::chrono::steady_clock::now();
-    while (!stopped && issued < max_launches) {
-        while (issued < max_launches && issued - completed < kInFlight) {
-            OF_TRY(launch_ls_iter(K, R, ph, b->mc, grid, b->hstatus, epoch, st));
-            issued++;
-        }
-        if (issued >= max_launches) break;
-        const unsigned long long word = __atomic_load_n(b->hstatus, __ATOMIC_ACQUIRE);
-        if ((unsigned)(word >> 32) == epoch) {
-            completed = (long long)(word & 0x7FFFFFFFull);
-            stopped = ((word >> 31) & 1ull) != 0;
-            if (stopped || issued - completed < kInFlight) continue;
-        }
-        if (++spins % 4096 == 0) {
-            const hipError_t q = hipStreamQuery(st);
-            if (q != hipSuccess && q != hipErrorNotReady) {
-                record_hip_error(q, "hipStreamQuery", __FILE__, __LINE__);
-                return OF_ERR_HIP;
-            }
-            if (q == hipSuccess) {
-                // idle stream: every issued launch has reported
-                const unsigned long long w2 = __atomic_load_n(b->hstatus, __ATOMIC_ACQUIRE);
-                if ((unsigned)(w2 >> 32) == epoch) {
-                    completed = (long long)(w2 & 0x7FFFFFFFull);
-                    stopped = ((w2 >> 31) & 1ull) != 0;
-                }
-                if (!stopped && completed < issued) return OF_ERR_STATE;
-            }
-            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > 60.0)
-                return OF_ERR_HIP;
-        } else {
-            __builtin_ia32_pause();
-        }
-    }
+    LsPhase ph = L.phases[L.s];
+    ph.warp = L.w;
+    L.phases[L.s].warp = L.w;
+    OF_TRY(launch_ls_warp_setup(ph, L.mc, L.stream));
+    L.grid = std::max(1u, std::min(ph.ntiles, it->second));
+    L.run_epoch = ++L.epoch;
+    L.max_launches = (b->prm.max_iterations + K - 1) / K + 1;
+    L.issued = L.completed = 0;
+    L.spins = 0;
+    L.t0 = std::chrono::steady_clock::now();
     return OF_OK;
 }
 
-// One lockstep group of n <= kLsMaxPairs pairs.
-int ls_group(of_tvl1_batch* b, int n, const int* widths, const int* heights,
-             const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
-             int* iterations_total)
+// One lockstep group of pairs `members` (at most kLsMaxPairs, with their
+// handles hs[]) on lane L: builds and uploads the tables, enqueues the
+// batched preparation and starts the coarsest scale.  ls_step does the rest.
+int ls_begin(const of_tvl1_batch* b, LsLane& L, const std::vector<int>& members,
+             of_tvl1* const* hall, const uint8_t* const* dI0, const uint8_t* const* dI1,
+             float* const* dU)
 {
     using namespace amdflow;
-    OF_TRY(ls_setup(b));
-    const int S = b->streams;
-    hipStream_t main = b->wstreams[0];
+    OF_TRY(ls_setup(L));
+    hipStream_t main = L.stream;
     const of_tvl1_params& P = b->prm;
-
-    // a handle per pair, pooled by frame size
-    std::vector<of_tvl1*> hs(n, nullptr);
-    {
-        std::map<std::pair<int, int>, size_t> used;
-        for (int i = 0; i < n; i++) {
-            const auto key = std::make_pair(widths[i], heights[i]);
-            auto& v = b->pool[key];
-            const size_t k = used[key]++;
-            while (v.size() <= k) {
-                of_tvl1* h = nullptr;
-                OF_TRY(of_tvl1_create(&P, widths[i], heights[i], &h));
-                v.push_back(h);
-            }
-            hs[i] = v[k];
-        }
-    }
-
-    // phase 1 of every pair on the batch's streams, joined into `main`
-    for (int i = 0; i < n; i++) {
-        of_tvl1* h = hs[i];
-        hipStream_t st = b->wstreams[i % S];
-        OF_TRY(tvl1_prepare(h, dI0[i], dI1[i], true, st));
-        const int c = h->ns - 1;   // zero flow at the coarsest scale, tvl1.cpp:199-200
-        OF_TRY(launch_fill(h->ubase + 4 * h->off[c], 2 * (size_t)h->nx[c] * h->ny[c], 0.0f, st));
-    }
-    for (int k = 1; k < S && k < n; k++) {
-        OF_HIP(hipEventRecord(b->wevents[k], b->wstreams[k]));
-        OF_HIP(hipStreamWaitEvent(main, b->wevents[k], 0));
-    }
+    const int n = (int)members.size();
+    L.members = members;
+    std::vector<of_tvl1*> hs(n);
+    for (int i = 0; i < n; i++) hs[i] = hall[members[i]];
 
     // tables: per scale, the pairs that have it
     int maxns = 0;
-    for (of_tvl1* h : hs) maxns = std::max(maxns, h->ns);
+    for (int i = 0; i < n; i++) maxns = std::max(maxns, hs[i]->ns);
     std::vector<std::vector<int>> part(maxns);
     size_t nent = 0, nbase = 0;
     for (int s = 0; s < maxns; s++) {
         for (int i = 0; i < n; i++)
             if (hs[i]->ns > s) part[s].push_back(i);
         nent += part[s].size();
-        nbase += 3 * (part[s].size() + 1);
+        // solve tables (3 prefix arrays) and preparation tables (participants
+        // and 2 prefix arrays at s >= 1; participants and 3 at s = 0)
+        nbase += 3 * (part[s].size() + 1) + part[s].size() + 3 * (part[s].size() + 1);
     }
-    const size_t niters = (size_t)n * kMaxScales * P.warps;
-    OF_TRY(ls_reserve(&b->h_pairs, &b->d_pairs, &b->pairs_cap, nent));
-    OF_TRY(ls_reserve(&b->h_bases, &b->d_bases, &b->bases_cap, nbase));
-    OF_TRY(ls_reserve(&b->h_iters, &b->d_iters, &b->iters_cap, niters));
+    L.niters = (size_t)n * kMaxScales * P.warps;
+    OF_TRY(ls_reserve(&L.h_preps, &L.d_preps, &L.preps_cap, (size_t)n));
+    OF_TRY(ls_reserve(&L.h_pairs, &L.d_pairs, &L.pairs_cap, nent));
+    OF_TRY(ls_reserve(&L.h_bases, &L.d_bases, &L.bases_cap, nbase));
+    OF_TRY(ls_reserve(&L.h_iters, &L.d_iters, &L.iters_cap, L.niters));
 
-    std::vector<LsPhase> phases(maxns);
-    std::vector<int> shapeK(maxns), shapeR(maxns);
+    L.phases.assign(maxns, LsPhase{});
+    L.shapeK.assign(maxns, 0);
+    L.shapeR.assign(maxns, 0);
     size_t e = 0, q = 0;
     for (int s = maxns - 1; s >= 0; s--) {
         const std::vector<int>& ps = part[s];
@@ -362,19 +329,19 @@ int ls_group(of_tvl1_batch* b, int n, const int* widths, const int* heights,
         for (int i : ps) maxpx = std::max(maxpx, (long long)hs[i]->nx[s] * hs[i]->ny[s]);
         int K, R;
         ls_shape(b, maxpx, &K, &R);
-        shapeK[s] = K;
-        shapeR[s] = R;
-        LsPhase& ph = phases[s];
-        ph.pairs = b->d_pairs + e;
-        ph.blk_base = b->d_bases + q;
-        ph.fblk_base = b->d_bases + q + (np + 1);
-        ph.tile_base = b->d_bases + q + 2 * (np + 1);
-        ph.iters = b->d_iters;
+        L.shapeK[s] = K;
+        L.shapeR[s] = R;
+        LsPhase& ph = L.phases[s];
+        ph.pairs = L.d_pairs + e;
+        ph.blk_base = L.d_bases + q;
+        ph.fblk_base = L.d_bases + q + (np + 1);
+        ph.tile_base = L.d_bases + q + 2 * (np + 1);
+        ph.iters = L.d_iters;
         ph.npairs = np;
         ph.level = s;
         ph.warps = P.warps;
         ph.warp = 0;
-        unsigned* blk = b->h_bases + q;
+        unsigned* blk = L.h_bases + q;
         unsigned* fblk = blk + (np + 1);
         unsigned* tiles = fblk + (np + 1);
         blk[0] = fblk[0] = tiles[0] = 0;
@@ -383,7 +350,7 @@ int ls_group(of_tvl1_batch* b, int n, const int* widths, const int* heights,
             of_tvl1* h = hs[i];
             const int nx = h->nx[s], ny = h->ny[s];
             const size_t N = (size_t)nx * ny;
-            LsPair& E = b->h_pairs[e + k];
+            LsPair& E = L.h_pairs[e + k];
             E = LsPair{};
             IterArgs& a = E.a;
             a.wx = h->wx;
@@ -409,7 +376,8 @@ int ls_group(of_tvl1_batch* b, int n, const int* widths, const int* heights,
             E.I1s = h->I1p + h->off[s];
             E.wsrc = reinterpret_cast<float4*>(h->wsrc);
             E.p0 = h->pbase;
-            E.dst = dU[i];
+            E.dst = dU[members[i]];
+            if (s == h->ns - 1) E.ucoarse = ub;   // zero flow, tvl1.cpp:199-200
             E.invz = 1 / P.zfactor;
             E.pair_id = (unsigned)i;
             unsigned gx, gy;
@@ -438,32 +406,151 @@ int ls_group(of_tvl1_batch* b, int n, const int* widths, const int* heights,
         e += np;
         q += 3 * (size_t)(np + 1);
     }
-    OF_HIP(hipMemcpyAsync(b->d_pairs, b->h_pairs, sizeof(LsPair) * nent, hipMemcpyHostToDevice, main));
-    OF_HIP(hipMemcpyAsync(b->d_bases, b->h_bases, sizeof(unsigned) * nbase, hipMemcpyHostToDevice, main));
-    OF_HIP(hipMemsetAsync(b->d_iters, 0, sizeof(int) * niters, main));
 
-    // phase 2: coarse to fine, all pairs per launch (tvl1.cpp:201-216)
-    for (int s = maxns - 1; s >= 0; s--) {
-        LsPhase ph = phases[s];
-        if (ph.npairs == 0) continue;
-        OF_TRY(launch_ls_level_begin(ph, main));
-        for (int w = 0; w < P.warps; w++) {
-            ph.warp = w;
-            OF_TRY(launch_ls_warp_setup(ph, b->mc, main));
-            OF_TRY(ls_run_warping(b, ph, shapeK[s], shapeR[s], main));
+    // preparation tables
+    for (int i = 0; i < n; i++) {
+        of_tvl1* h = hs[i];
+        LsPrep& R = L.h_preps[i];
+        R = LsPrep{};
+        R.s0 = dI0[members[i]];
+        R.s1 = dI1[members[i]];
+        R.A0 = h->A0;
+        R.A1 = h->A1;
+        R.B0 = h->wx;
+        R.B1 = h->wy;
+        R.I0p = h->I0p;
+        R.I1p = h->I1p;
+        R.ctl = h->ctl;
+        R.n0 = (unsigned)((size_t)h->W * h->H);
+        R.nmm = std::min((unsigned)cdiv(R.n0, 256), (unsigned)kMinMaxBlocks);
+        R.ns = h->ns;
+        for (int s = 0; s < h->ns; s++) {
+            R.nx[s] = h->nx[s];
+            R.ny[s] = h->ny[s];
+            R.off[s] = (unsigned)h->off[s];
         }
-        if (s > 0) OF_TRY(launch_ls_zoom_in(ph, main));
     }
-    OF_TRY(launch_ls_copy_flow(phases[0], main));
-    OF_HIP(hipMemcpyAsync(b->h_iters, b->d_iters, sizeof(int) * niters, hipMemcpyDeviceToHost, main));
-    OF_HIP(hipStreamSynchronize(main));
-    if (iterations_total)
-        for (int i = 0; i < n; i++) {
-            long long sum = 0;
-            for (size_t k = 0; k < (size_t)kMaxScales * P.warps; k++)
-                sum += b->h_iters[(size_t)i * kMaxScales * P.warps + k];
-            iterations_total[i] = (int)sum;
+    // one step: participants and the prefix sums of their item counts
+    auto add_step = [&](const std::vector<int>& ps, const unsigned* part_dev, int s,
+                        auto&& items) -> LsPrepStep {
+        LsPrepStep st{};
+        st.preps = L.d_preps;
+        st.part = part_dev;
+        st.base = L.d_bases + q;
+        st.npart = (int)ps.size();
+        st.s = s;
+        unsigned* base = L.h_bases + q;
+        base[0] = 0;
+        for (size_t k = 0; k < ps.size(); k++) base[k + 1] = base[k] + items(hs[ps[k]]);
+        st.nitems = base[ps.size()];
+        q += ps.size() + 1;
+        return st;
+    };
+    auto blocks = [](int nx, int ny) { return (unsigned)(cdiv((size_t)nx, 64) * cdiv((size_t)ny, 4)); };
+    std::vector<const unsigned*> part_dev(maxns);
+    for (int s = 0; s < maxns; s++) {
+        part_dev[s] = L.d_bases + q;
+        for (size_t k = 0; k < part[s].size(); k++) L.h_bases[q + k] = (unsigned)part[s][k];
+        q += part[s].size();
+    }
+    const LsPrepStep st_load = add_step(part[0], part_dev[0], 0, [&](of_tvl1* h) {
+        return std::min((unsigned)cdiv((size_t)h->W * h->H, 256), (unsigned)kMinMaxBlocks);
+    });
+    const LsPrepStep st_norm = add_step(part[0], part_dev[0], 0,
+                                        [&](of_tvl1* h) { return (unsigned)cdiv((size_t)h->W * h->H, 256); });
+    std::vector<LsPrepStep> st_blur(maxns), st_zoom(maxns);
+    for (int s = 0; s < maxns; s++) {
+        const int sc = s > 0 ? s - 1 : 0;
+        st_blur[s] = add_step(part[s], part_dev[s], s,
+                              [&](of_tvl1* h) { return 2 * blocks(h->nx[sc], h->ny[sc]); });
+        if (s > 0)
+            st_zoom[s] = add_step(part[s], part_dev[s], s,
+                                  [&](of_tvl1* h) { return 2 * blocks(h->nx[s], h->ny[s]); });
+    }
+    if (q > nbase) return OF_ERR_STATE;   // table sizing above is wrong
+
+    OF_HIP(hipMemcpyAsync(L.d_preps, L.h_preps, sizeof(LsPrep) * n, hipMemcpyHostToDevice, main));
+    OF_HIP(hipMemcpyAsync(L.d_pairs, L.h_pairs, sizeof(LsPair) * nent, hipMemcpyHostToDevice, main));
+    OF_HIP(hipMemcpyAsync(L.d_bases, L.h_bases, sizeof(unsigned) * q, hipMemcpyHostToDevice, main));
+    OF_HIP(hipMemsetAsync(L.d_iters, 0, sizeof(int) * L.niters, main));
+
+    // phase 1 of all pairs (tvl1.cpp:181-196): frames to float with the
+    // min/max keys, control blocks, normalisation, pre-smoothing, pyramids
+    const GaussTaps& g_pre = hs[0]->g_pre;    // the same for every handle of the batch
+    const GaussTaps& g_zoom = hs[0]->g_zoom;
+    OF_TRY(launch_lp_load(st_load, main));
+    OF_TRY(launch_lp_final(L.d_preps, n, main));
+    OF_TRY(launch_lp_normalize(st_norm, main));
+    OF_TRY(launch_lp_blur(st_blur[0], g_pre, main));
+    for (int s = 1; s < maxns; s++) {
+        OF_TRY(launch_lp_blur(st_blur[s], g_zoom, main));
+        OF_TRY(launch_lp_zoom_out(st_zoom[s], P.zfactor, main));
+    }
+
+    // phase 2 starts at the coarsest scale (tvl1.cpp:201-216)
+    L.done = false;
+    L.s = maxns - 1;
+    L.w = 0;
+    OF_TRY(launch_ls_level_begin(L.phases[L.s], main));
+    return ls_start_warping(b, L);
+}
+
+// Advances lane L without waiting: keeps kInFlight iteration launches
+// queued; when the device reports that every pair of the warping stopped
+// (or the launch bound is reached), enqueues the next warping, scale or the
+// group's end.  *progress is set when something was enqueued or completed.
+int ls_step(const of_tvl1_batch* b, LsLane& L, bool* progress)
+{
+    using namespace amdflow;
+    if (L.done) return OF_OK;
+    const int K = L.shapeK[L.s], R = L.shapeR[L.s];
+    bool stopped = false;
+    const unsigned long long word = __atomic_load_n(L.hstatus, __ATOMIC_ACQUIRE);
+    if ((unsigned)(word >> 32) == L.run_epoch) {
+        const long long c = (long long)(word & 0x7FFFFFFFull);
+        if (c != L.completed) *progress = true;
+        L.completed = c;
+        stopped = ((word >> 31) & 1ull) != 0;
+    }
+    if (!stopped) {
+        while (L.issued < L.max_launches && L.issued - L.completed < kInFlight) {
+            OF_TRY(launch_ls_iter(K, R, L.phases[L.s], L.mc, L.grid, L.hstatus, L.run_epoch, L.stream));
+            L.issued++;
+            *progress = true;
         }
+        if (L.issued < L.max_launches) {
+            if (++L.spins % 4096 == 0) {
+                const hipError_t q = hipStreamQuery(L.stream);
+                if (q != hipSuccess && q != hipErrorNotReady) {
+                    record_hip_error(q, "hipStreamQuery", __FILE__, __LINE__);
+                    return OF_ERR_HIP;
+                }
+                if (q == hipSuccess) {
+                    // idle stream: every issued launch has reported
+                    const unsigned long long w2 = __atomic_load_n(L.hstatus, __ATOMIC_ACQUIRE);
+                    const bool ok = (unsigned)(w2 >> 32) == L.run_epoch &&
+                                    (((w2 >> 31) & 1ull) || (long long)(w2 & 0x7FFFFFFFull) >= L.issued);
+                    if (!ok) return OF_ERR_STATE;
+                }
+                if (std::chrono::duration<double>(std::chrono::steady_clock::now() - L.t0).count() > 60.0)
+                    return OF_ERR_HIP;
+            }
+            return OF_OK;   // the warping is still running
+        }
+    }
+    // the warping is over (launches still queued after the stop are no-ops)
+    *progress = true;
+    const of_tvl1_params& P = b->prm;
+    if (++L.w < P.warps) return ls_start_warping(b, L);
+    if (L.s > 0) OF_TRY(launch_ls_zoom_in(L.phases[L.s], L.stream));
+    if (--L.s >= 0) {
+        L.w = 0;
+        OF_TRY(launch_ls_level_begin(L.phases[L.s], L.stream));
+        return ls_start_warping(b, L);
+    }
+    OF_TRY(launch_ls_copy_flow(L.phases[0], L.stream));
+    OF_HIP(hipMemcpyAsync(L.h_iters, L.d_iters, sizeof(int) * L.niters, hipMemcpyDeviceToHost, L.stream));
+    L.done = true;
     return OF_OK;
 }
 
@@ -471,11 +558,78 @@ int run_lockstep(of_tvl1_batch* b, int npairs, const int* widths, const int* hei
                  const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
                  int* iterations_total)
 {
+    if (npairs == 0) return OF_OK;
     if (hipSetDevice(b->device) != hipSuccess) return OF_ERR_HIP;
-    for (int g = 0; g < npairs; g += amdflow::kLsMaxPairs) {
-        const int n = std::min(amdflow::kLsMaxPairs, npairs - g);
-        OF_TRY(ls_group(b, n, widths + g, heights + g, dI0 + g, dI1 + g, dU + g,
-                        iterations_total ? iterations_total + g : nullptr));
+    const int nl = std::max(1, std::min(b->streams, npairs));
+    while ((int)b->lanes.size() < nl) {
+        b->lanes.emplace_back();
+        b->lanes.back().stream = b->wstreams[b->lanes.size() - 1];
+    }
+    // a handle per pair, pooled by frame size
+    std::vector<of_tvl1*> hs(npairs, nullptr);
+    {
+        std::map<std::pair<int, int>, size_t> used;
+        for (int i = 0; i < npairs; i++) {
+            const auto key = std::make_pair(widths[i], heights[i]);
+            auto& v = b->pool[key];
+            const size_t k = used[key]++;
+            while (v.size() <= k) {
+                of_tvl1* h = nullptr;
+                OF_TRY(of_tvl1_create(&b->prm, widths[i], heights[i], &h));
+                v.push_back(h);
+            }
+            hs[i] = v[k];
+        }
+    }
+    // pairs to lanes, largest first onto the lane with the fewest pixels
+    std::vector<int> order(npairs);
+    for (int i = 0; i < npairs; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (long long)widths[x] * heights[x] > (long long)widths[y] * heights[y];
+    });
+    std::vector<std::vector<int>> mine(nl);
+    std::vector<long long> load(nl, 0);
+    for (int i : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        mine[l].push_back(i);
+        load[l] += (long long)widths[i] * heights[i];
+    }
+    for (auto& m : mine) std::sort(m.begin(), m.end());
+    // rounds of at most kLsMaxPairs pairs per lane, all lanes advanced in turn
+    for (size_t g = 0;; g += kLsMaxPairsPerGroup) {
+        bool any = false;
+        for (int l = 0; l < nl; l++) {
+            LsLane& L = b->lanes[l];
+            L.done = true;
+            if (g >= mine[l].size()) continue;
+            const size_t n = std::min(mine[l].size() - g, kLsMaxPairsPerGroup);
+            const std::vector<int> members(mine[l].begin() + g, mine[l].begin() + g + n);
+            OF_TRY(ls_begin(b, L, members, hs.data(), dI0, dI1, dU));
+            any = true;
+        }
+        if (!any) break;
+        for (;;) {
+            bool all = true, progress = false;
+            for (int l = 0; l < nl; l++) {
+                LsLane& L = b->lanes[l];
+                OF_TRY(ls_step(b, L, &progress));
+                all = all && L.done;
+            }
+            if (all) break;
+            if (!progress) __builtin_ia32_pause();
+        }
+        for (int l = 0; l < nl; l++) {
+            LsLane& L = b->lanes[l];
+            if (L.members.empty() || g >= mine[l].size()) continue;
+            OF_HIP(hipStreamSynchronize(L.stream));
+            if (iterations_total)
+                for (size_t k = 0; k < L.members.size(); k++) {
+                    long long sum = 0;
+                    for (size_t j = 0; j < (size_t)amdflow::kMaxScales * b->prm.warps; j++)
+                        sum += L.h_iters[k * amdflow::kMaxScales * b->prm.warps + j];
+                    iterations_total[L.members[k]] = (int)sum;
+                }
+        }
     }
     return OF_OK;
 }

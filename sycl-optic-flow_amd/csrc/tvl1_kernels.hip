@@ -48,26 +48,6 @@ __global__ void k_ctl_begin(TvCtl* ctl)
 // input conversion + joint min/max (tvl1.cpp:160-165, 352-361)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
 // Converts both frames to float and finds, per frame, the first index of the
 // largest and of the smallest |x| (cblas_isamax / cblas_isamin semantics).
 template <typename T>
@@ -82,13 +62,7 @@ __global__ __launch_bounds__(256) void k_load_minmax(const T* __restrict__ a,
         const float va = (float)a[i], vb = (float)b[i];
         fa[i] = va;
         fb[i] = vb;
-        const unsigned long long ka = (unsigned long long)__float_as_uint(fabsf(va)) << 32;
-        const unsigned long long kb = (unsigned long long)__float_as_uint(fabsf(vb)) << 32;
-        const unsigned long long hi = 0xFFFFFFFFull - i;
-        kmax0 = (ka | hi) > kmax0 ? (ka | hi) : kmax0;
-        kmax1 = (kb | hi) > kmax1 ? (kb | hi) : kmax1;
-        kmin0 = (ka | i) < kmin0 ? (ka | i) : kmin0;
-        kmin1 = (kb | i) < kmin1 ? (kb | i) : kmin1;
+        minmax_keys(va, vb, i, kmax0, kmax1, kmin0, kmin1);
     }
     kmax0 = wave_max_u64(kmax0);
     kmax1 = wave_max_u64(kmax1);
@@ -159,34 +133,17 @@ __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ a,
                                                    float* __restrict__ ob, unsigned n,
                                                    const TvCtl* ctl)
 {
-    const unsigned imax0 = 0xFFFFFFFFu - (unsigned)(ctl->maxkey[0] & 0xFFFFFFFFull);
-    const unsigned imax1 = 0xFFFFFFFFu - (unsigned)(ctl->maxkey[1] & 0xFFFFFFFFull);
-    const unsigned imin0 = (unsigned)(ctl->minkey[0] & 0xFFFFFFFFull);
-    const unsigned imin1 = (unsigned)(ctl->minkey[1] & 0xFFFFFFFFull);
-    const float a0 = a[imax0], a1 = b[imax1];
-    const float b0 = a[imin0], b1 = b[imin1];
-    const float mx = (a0 < a1) ? a1 : a0;
-    const float mn = (b1 < b0) ? b1 : b0;
-    const float den = mx - mn;
+    float mn, den;
+    norm_bounds(ctl, a, b, &mn, &den);
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (den <= 0) {
-            oa[i] = a[i];
-            ob[i] = b[i];
-        } else {
-            oa[i] = (float)(255.0 * (double)(a[i] - mn) / (double)den);
-            ob[i] = (float)(255.0 * (double)(b[i] - mn) / (double)den);
-        }
+        oa[i] = norm_px(a[i], mn, den);
+        ob[i] = norm_px(b[i], mn, den);
     }
 }
 
 // ---------------------------------------------------------------------------
 // separable Gaussian (mask.cpp:217-287)
 // ---------------------------------------------------------------------------
-
-// Low side: whole-sample mirror x[-k] = x[k]; high side: half-sample mirror
-// x[n-1+k] = x[n-k] (mask.cpp:253-255, 274-276).
-__device__ __forceinline__ int mirror_lo(int i) { return i < 0 ? -i : i; }
-__device__ __forceinline__ int mirror_hi(int i, int n) { return i >= n ? 2 * n - 1 - i : i; }
 
 __global__ __launch_bounds__(256) void k_gauss_rows(const float* __restrict__ src,
                                                     float* __restrict__ dst, int nx, int ny,
@@ -195,13 +152,7 @@ __global__ __launch_bounds__(256) void k_gauss_rows(const float* __restrict__ sr
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= nx || y >= ny) return;
-    const float* row = src + (size_t)y * nx;
-    double sum = g.B[0] * (double)row[x];
-    for (int j = 1; j < g.size; j++) {
-        const float pair = row[mirror_lo(x - j)] + row[mirror_hi(x + j, nx)];
-        sum += g.B[j] * (double)pair;
-    }
-    dst[(size_t)y * nx + x] = (float)sum;
+    dst[(size_t)y * nx + x] = gauss_row_px(src, nx, x, y, g);
 }
 
 __global__ __launch_bounds__(256) void k_gauss_cols(const float* __restrict__ src,
@@ -211,13 +162,7 @@ __global__ __launch_bounds__(256) void k_gauss_cols(const float* __restrict__ sr
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= nx || y >= ny) return;
-    const float* col = src + x;
-    double sum = g.B[0] * (double)col[(size_t)y * nx];
-    for (int j = 1; j < g.size; j++) {
-        const float pair = col[(size_t)mirror_lo(y - j) * nx] + col[(size_t)mirror_hi(y + j, ny) * nx];
-        sum += g.B[j] * (double)pair;
-    }
-    dst[(size_t)y * nx + x] = (float)sum;
+    dst[(size_t)y * nx + x] = gauss_col_px(src, nx, ny, x, y, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,9 +177,7 @@ __global__ __launch_bounds__(256) void k_zoom_out_sample(const float* __restrict
     const int j1 = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i1 = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j1 >= nxx || i1 >= nyy) return;
-    const float i2 = (float)i1 / factor;
-    const float j2 = (float)j1 / factor;
-    dst[(size_t)i1 * nxx + j1] = bicubic_at(src, j2, i2, nx, ny, false);
+    dst[(size_t)i1 * nxx + j1] = zoom_out_px(src, nx, ny, i1, j1, factor);
 }
 
 // zoom_in of both flow components (zoom.cpp:61-85) fused with the

@@ -5,7 +5,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
 #include "device_math.hpp"
+#include "tvl1_kernels.hpp"
 
 namespace amdflow {
 
@@ -62,6 +64,109 @@ __device__ __forceinline__ void zoom_in_flow_px(const float* __restrict__ uc,
     const float b = bicubic_sample(uc + nc, nx, t);
     uf[(size_t)i1 * nxx + j1] = invz * a;
     uf[nf + (size_t)i1 * nxx + j1] = invz * b;
+}
+
+// ---------------------------------------------------------------------------
+// phase 1: normalisation, Gaussian, pyramid sampling
+// ---------------------------------------------------------------------------
+
+// image_normalization bounds (tvl1.cpp:344-371) from the min/max keys of
+// both frames: mx = max of the two isamax picks, mn = min of the two isamin
+// picks, den = mx - mn.
+__device__ __forceinline__ void norm_bounds(const TvCtl* ctl, const float* a, const float* b,
+                                            float* mn, float* den)
+{
+    const unsigned imax0 = 0xFFFFFFFFu - (unsigned)(ctl->maxkey[0] & 0xFFFFFFFFull);
+    const unsigned imax1 = 0xFFFFFFFFu - (unsigned)(ctl->maxkey[1] & 0xFFFFFFFFull);
+    const unsigned imin0 = (unsigned)(ctl->minkey[0] & 0xFFFFFFFFull);
+    const unsigned imin1 = (unsigned)(ctl->minkey[1] & 0xFFFFFFFFull);
+    const float a0 = a[imax0], a1 = b[imax1];
+    const float b0 = a[imin0], b1 = b[imin1];
+    const float mx = (a0 < a1) ? a1 : a0;
+    *mn = (b1 < b0) ? b1 : b0;
+    *den = mx - *mn;
+}
+
+// One normalised value: unchanged when den <= 0, else 255*(x-mn)/den in double.
+__device__ __forceinline__ float norm_px(float x, float mn, float den)
+{
+    return den <= 0 ? x : (float)(255.0 * (double)(x - mn) / (double)den);
+}
+
+// cblas_isamax / cblas_isamin of both frames as 64-bit keys: |x| bits in
+// the high word, the index (complemented for the max, so that the first
+// index wins) in the low word.
+__device__ __forceinline__ void minmax_keys(float va, float vb, unsigned i, unsigned long long& kmax0,
+                                            unsigned long long& kmax1, unsigned long long& kmin0,
+                                            unsigned long long& kmin1)
+{
+    const unsigned long long ka = (unsigned long long)__float_as_uint(fabsf(va)) << 32;
+    const unsigned long long kb = (unsigned long long)__float_as_uint(fabsf(vb)) << 32;
+    const unsigned long long hi = 0xFFFFFFFFull - i;
+    kmax0 = (ka | hi) > kmax0 ? (ka | hi) : kmax0;
+    kmax1 = (kb | hi) > kmax1 ? (kb | hi) : kmax1;
+    kmin0 = (ka | i) < kmin0 ? (ka | i) : kmin0;
+    kmin1 = (kb | i) < kmin1 ? (kb | i) : kmin1;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Low side: whole-sample mirror x[-k] = x[k]; high side: half-sample mirror
+// x[n-1+k] = x[n-k] (mask.cpp:253-255, 274-276).
+__device__ __forceinline__ int mirror_lo(int i) { return i < 0 ? -i : i; }
+__device__ __forceinline__ int mirror_hi(int i, int n) { return i >= n ? 2 * n - 1 - i : i; }
+
+// Separable Gaussian (mask.cpp:217-287), double accumulation: the row pass
+// and the column pass at pixel (y, x).
+__device__ __forceinline__ float gauss_row_px(const float* __restrict__ src, int nx, int x, int y,
+                                              const GaussTaps& g)
+{
+    const float* row = src + (size_t)y * nx;
+    double sum = g.B[0] * (double)row[x];
+    for (int j = 1; j < g.size; j++) {
+        const float pair = row[mirror_lo(x - j)] + row[mirror_hi(x + j, nx)];
+        sum += g.B[j] * (double)pair;
+    }
+    return (float)sum;
+}
+
+__device__ __forceinline__ float gauss_col_px(const float* __restrict__ src, int nx, int ny, int x,
+                                              int y, const GaussTaps& g)
+{
+    const float* col = src + x;
+    double sum = g.B[0] * (double)col[(size_t)y * nx];
+    for (int j = 1; j < g.size; j++) {
+        const float pair = col[(size_t)mirror_lo(y - j) * nx] + col[(size_t)mirror_hi(y + j, ny) * nx];
+        sum += g.B[j] * (double)pair;
+    }
+    return (float)sum;
+}
+
+// zoom_out's resampling of the blurred image at (j/f, i/f), zoom.cpp:28-52.
+__device__ __forceinline__ float zoom_out_px(const float* __restrict__ src, int nx, int ny, int i1,
+                                             int j1, float factor)
+{
+    const float i2 = (float)i1 / factor;
+    const float j2 = (float)j1 / factor;
+    return bicubic_at(src, j2, i2, nx, ny, false);
 }
 
 }  // namespace amdflow

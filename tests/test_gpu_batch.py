@@ -32,13 +32,14 @@ def _run(b, sizes, d0, d1):
     return it, [t.cpu().numpy() for t in dU]
 
 
-@pytest.mark.parametrize("mode", ["lockstep", "workers"])
-def test_batch_matches_oracle(mode):
+@pytest.mark.parametrize("mode,streams", [("lockstep", 1), ("lockstep", 3), ("workers", 4)])
+def test_batch_matches_oracle(mode, streams):
+    """Lockstep with one lane and with three lanes; the worker mode."""
     from amdflow.batch import TVL1Batch
     # mixed sizes with 3, 4 and 5 scales (default nscales rule), two of one size
     sizes = [(160, 120), (200, 150), (320, 240), (97, 61), (160, 120), (256, 192), (320, 240)]
     frames, d0, d1 = _frames(sizes, 3.0, 100)
-    with TVL1Batch(streams=4, mode=mode) as b:
+    with TVL1Batch(streams=streams, mode=mode) as b:
         it, flows = _run(b, sizes, d0, d1)
     for k, (q0, q1) in enumerate(frames):
         uo, ito = O.tvl1(q0, q1)
@@ -82,7 +83,7 @@ def test_lockstep_launch_shapes_agree(K, R):
     from amdflow.batch import TVL1Batch
     sizes = [(160, 120), (97, 61), (200, 150)]
     frames, d0, d1 = _frames(sizes, 3.0, 500)
-    with TVL1Batch(streams=2) as b:
+    with TVL1Batch(streams=1) as b:
         b.set_shape(0, K, R)
         it, flows = _run(b, sizes, d0, d1)
     for k, (q0, q1) in enumerate(frames):
