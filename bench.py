@@ -48,7 +48,8 @@ def parse():
     ap.add_argument("--retune", action="store_true", help="re-run the launch-shape autotuner")
     ap.add_argument("--batch-pairs", type=int, default=64,
                     help="pairs of the batched mixed-size measurement (0 = skip)")
-    ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--batch-lanes", type=int, default=3,
+                    help="lanes (concurrent streams) of the lockstep batch solver")
     ap.add_argument("--no-other", action="store_true", help="skip the HS and LK measurements")
     ap.add_argument("--warm-pairs", type=int, default=16,
                     help="pairs of the warm-start measurement (0 = skip)")
@@ -76,20 +77,23 @@ def calibrate_copy():
 BATCH_SIZES = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
 
 
-def batched(npairs: int, streams: int, reps: int):
+def batched(npairs: int, lanes: int, reps: int):
     """64 pairs whose sizes are drawn uniformly (seeded) from BATCH_SIZES,
     same generator and parameters as the 1920x1080 workload, solved by the
-    batched solver; two generated pairs per size are cycled through."""
+    lockstep batch solver (and, for comparison, its worker mode); two
+    generated pairs per size are cycled through.  The algorithmic bytes of
+    the batch are the sum of its pairs' (each pair solved once alone for its
+    iteration counts, which the batch reproduces exactly)."""
     import numpy as np
     import torch
 
-    from amdflow import synth
+    from amdflow import TV_L1, synth
     from amdflow.batch import TVL1Batch
 
     rng = np.random.default_rng(SEED + 1)
     pick = rng.integers(0, len(BATCH_SIZES), npairs)
-    cache = {}
-    d0, d1, dU, ws, hs = [], [], [], [], []
+    cache, alg = {}, {}
+    d0, d1, dU, ws, hs, keys = [], [], [], [], [], []
     for k, s in enumerate(pick):
         w, h = BATCH_SIZES[int(s)]
         key = (w, h, k % 2)
@@ -100,22 +104,41 @@ def batched(npairs: int, streams: int, reps: int):
         a, b = cache[key]
         d0.append(a.data_ptr()); d1.append(b.data_ptr())
         dU.append(torch.empty(2 * w * h, dtype=torch.float32, device="cuda"))
-        ws.append(w); hs.append(h)
+        ws.append(w); hs.append(h); keys.append(key)
+    for key, (a, b) in cache.items():
+        tv = TV_L1(key[0], key[1], nscales=5)
+        out = torch.empty(2 * key[0] * key[1], dtype=torch.float32, device="cuda")
+        tv.run_device_u8(a.data_ptr(), b.data_ptr(), out.data_ptr())
+        st = tv.stats()
+        alg[key] = (st["algorithmic_bytes"], int(tv.iterations().sum()))
+        tv.close()
     up = [t.data_ptr() for t in dU]
-    with TVL1Batch(streams=streams, nscales=5) as b:
-        b.run_device_u8(ws, hs, d0, d1, up)                 # warm-up (handles, caches)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            its = b.run_device_u8(ws, hs, d0, d1, up)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
+
+    def timed(mode, streams):
+        with TVL1Batch(streams=streams, nscales=5, mode=mode) as b:
+            b.run_device_u8(ws, hs, d0, d1, up)                 # warm-up (handles, tables)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                its = b.run_device_u8(ws, hs, d0, d1, up)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps, its
+
+    dt, its = timed("lockstep", lanes)
+    dt_w, its_w = timed("workers", 4)
+    same = all(int(its[i]) == alg[k][1] for i, k in enumerate(keys)) and bool(np.array_equal(its, its_w))
+    B = sum(alg[k][0] for k in keys)
     mpx = sum(w * h for w, h in zip(ws, hs)) / 1e6
-    return {"workload": f"tvl1_batch{npairs}_mixed", "pairs": npairs, "streams": streams,
-            "sizes": [f"{w}x{h}" for w, h in BATCH_SIZES],
+    return {"workload": f"tvl1_batch{npairs}_mixed", "pairs": npairs, "mode": "lockstep",
+            "lanes": lanes, "sizes": [f"{w}x{h}" for w, h in BATCH_SIZES],
             "pair_counts": [int((pick == i).sum()) for i in range(len(BATCH_SIZES))],
             "ms_per_batch": round(dt * 1e3, 2), "pairs_per_s": round(npairs / dt, 2),
             "mpixel_per_s": round(mpx / dt, 1), "iterations_total": int(np.sum(its)),
+            "algorithmic_bytes": B, "algorithmic_GBps": round(B / dt / 1e9, 1),
+            "hbm_roofline_frac": round(B / dt / 8e12, 4),
+            "workers_mode": {"streams": 4, "ms_per_batch": round(dt_w * 1e3, 2),
+                             "hbm_roofline_frac": round(B / dt_w / 8e12, 4)},
+            "iterations_match_single_solves": same,
             "data": "synthetic, 2 generated pairs per size cycled through the batch"}
 
 
@@ -371,7 +394,7 @@ def main():
         "roofline": roofline,
     }
     if args.batch_pairs > 0:
-        result["batched"] = batched(args.batch_pairs, args.batch_streams, reps=3)
+        result["batched"] = batched(args.batch_pairs, args.batch_lanes, reps=3)
     if args.warm_pairs > 0:
         result["warm_start"] = warm_start(args.warm_pairs)
     if not args.no_other:

@@ -41,6 +41,21 @@ constexpr int kEnergyBlocks = 1024;
 
 namespace {
 
+// The handle's own stream, for the host-pointer calls and the autotuner,
+// created on first use: every stream holds one of the device's few hardware
+// queues (4 by default), and idle streams created before a batch's lanes
+// make the lanes share queues.
+int own_stream(of_tvl1* h, hipStream_t* out)
+{
+    if (!h->stream && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->stream = nullptr;
+        record_hip_error(hipErrorOutOfMemory, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+        return OF_ERR_HIP;
+    }
+    *out = h->stream;
+    return OF_OK;
+}
+
 int pick_rule(const of_tvl1* h, long long npx, int* K, int* R, int* persistent)
 {
     const LaunchRule* best = nullptr;
@@ -370,6 +385,8 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
         h->scale_ev.resize(2 * kMaxScales);
         for (hipEvent_t& e : h->scale_ev) OF_HIP(hipEventCreate(&e));
     }
+    hipStream_t hs = nullptr;
+    OF_TRY(own_stream(h, &hs));
     std::vector<double> best(h->ns, 1e30);
     std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8), bestP(h->ns, 0);
     const std::vector<LaunchRule> saved = h->rules;
@@ -377,7 +394,7 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
         for (int c = 0; c < nshape; c++) {
             const int row = mode * nshape + c;
             h->rules.assign(1, LaunchRule{0, kShapes[c][0], kShapes[c][1], mode});
-            int st = solve(h, d0, d1, true, h->dflow, h->stream);   // warm-up of this variant
+            int st = solve(h, d0, d1, true, h->dflow, hs);   // warm-up of this variant
             // a persistent variant the device cannot run (no cooperative
             // launch) or whose grid barrier ran past its bound is skipped;
             // the kernels exit in both cases and the device stays usable
@@ -388,7 +405,7 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
             }
             for (int s = 0; s < h->ns; s++) h->scale_ms[s] = 0.0;
             h->scale_timing = true;
-            for (int r = 0; r < reps && st == OF_OK; r++) st = solve(h, d0, d1, true, h->dflow, h->stream);
+            for (int r = 0; r < reps && st == OF_OK; r++) st = solve(h, d0, d1, true, h->dflow, hs);
             h->scale_timing = false;
             if (st != OF_OK) {
                 h->rules = saved;
@@ -501,11 +518,6 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
                     maxpart = std::max(maxpart, (size_t)gx * gy * K);
                 }
     
-        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-            h->stream = nullptr;
-            release(h);
-            return OF_ERR_HIP;
-        }
         st = OF_OK;
         if (!st) st = dev_alloc(&h->I0p, h->total);
         if (!st) st = dev_alloc(&h->I1p, h->total);
@@ -539,8 +551,7 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         *h->hstatus = 0ull;
         std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
         h->h_iters.assign((size_t)ns * p.warps, 0);
-        if (hipMemsetAsync(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps, h->stream) != hipSuccess ||
-            hipStreamSynchronize(h->stream) != hipSuccess) {
+        if (hipMemset(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps) != hipSuccess) {
             release(h);
             return OF_ERR_HIP;
         }
@@ -699,12 +710,14 @@ int of_tvl1_run_u8(of_tvl1* h, const uint8_t* I0, const uint8_t* I1)
 {
     return guarded([&]() -> int {
         if (!h || !I0 || !I1) return OF_ERR_ARG;
+        hipStream_t hs = nullptr;
+        OF_TRY(own_stream(h, &hs));
         const size_t N0 = (size_t)h->W * h->H;
-        OF_HIP(hipMemcpyAsync(h->in8, I0, N0, hipMemcpyHostToDevice, h->stream));
-        OF_HIP(hipMemcpyAsync(h->in8 + N0, I1, N0, hipMemcpyHostToDevice, h->stream));
-        OF_TRY(solve(h, h->in8, h->in8 + N0, true, h->dflow, h->stream));
-        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
-        OF_HIP(hipStreamSynchronize(h->stream));
+        OF_HIP(hipMemcpyAsync(h->in8, I0, N0, hipMemcpyHostToDevice, hs));
+        OF_HIP(hipMemcpyAsync(h->in8 + N0, I1, N0, hipMemcpyHostToDevice, hs));
+        OF_TRY(solve(h, h->in8, h->in8 + N0, true, h->dflow, hs));
+        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, hs));
+        OF_HIP(hipStreamSynchronize(hs));
         return OF_OK;
     });
 }
@@ -713,12 +726,14 @@ int of_tvl1_run_f32(of_tvl1* h, const float* I0, const float* I1)
 {
     return guarded([&]() -> int {
         if (!h || !I0 || !I1) return OF_ERR_ARG;
+        hipStream_t hs = nullptr;
+        OF_TRY(own_stream(h, &hs));
         const size_t N0 = (size_t)h->W * h->H;
-        OF_HIP(hipMemcpyAsync(h->inf, I0, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-        OF_HIP(hipMemcpyAsync(h->inf + N0, I1, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-        OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
-        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
-        OF_HIP(hipStreamSynchronize(h->stream));
+        OF_HIP(hipMemcpyAsync(h->inf, I0, sizeof(float) * N0, hipMemcpyHostToDevice, hs));
+        OF_HIP(hipMemcpyAsync(h->inf + N0, I1, sizeof(float) * N0, hipMemcpyHostToDevice, hs));
+        OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, hs));
+        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, hs));
+        OF_HIP(hipStreamSynchronize(hs));
         return OF_OK;
     });
 }
@@ -727,21 +742,23 @@ int of_tvl1_push_f32(of_tvl1* h, const float* I)
 {
     return guarded([&]() -> int {
         if (!h || !I) return OF_ERR_ARG;
+        hipStream_t hs = nullptr;
+        OF_TRY(own_stream(h, &hs));
         const size_t N0 = (size_t)h->W * h->H;
         if (!h->have_prev) {
-            OF_HIP(hipMemcpyAsync(h->prev, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-            OF_HIP(hipStreamSynchronize(h->stream));
+            OF_HIP(hipMemcpyAsync(h->prev, I, sizeof(float) * N0, hipMemcpyHostToDevice, hs));
+            OF_HIP(hipStreamSynchronize(hs));
             std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
             h->have_prev = true;
             return OF_OK;
         }
         // I0 = previous frame, I1 = this frame (src/tvl1/cuda/tvl1.cu:126-130)
-        OF_HIP(hipMemcpyAsync(h->inf, h->prev, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
-        OF_HIP(hipMemcpyAsync(h->inf + N0, I, sizeof(float) * N0, hipMemcpyHostToDevice, h->stream));
-        OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, h->stream));
-        OF_HIP(hipMemcpyAsync(h->prev, h->inf + N0, sizeof(float) * N0, hipMemcpyDeviceToDevice, h->stream));
-        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, h->stream));
-        OF_HIP(hipStreamSynchronize(h->stream));
+        OF_HIP(hipMemcpyAsync(h->inf, h->prev, sizeof(float) * N0, hipMemcpyDeviceToDevice, hs));
+        OF_HIP(hipMemcpyAsync(h->inf + N0, I, sizeof(float) * N0, hipMemcpyHostToDevice, hs));
+        OF_TRY(solve(h, h->inf, h->inf + N0, false, h->dflow, hs));
+        OF_HIP(hipMemcpyAsync(h->prev, h->inf + N0, sizeof(float) * N0, hipMemcpyDeviceToDevice, hs));
+        OF_HIP(hipMemcpyAsync(h->hflow, h->dflow, sizeof(float) * 2 * N0, hipMemcpyDeviceToHost, hs));
+        OF_HIP(hipStreamSynchronize(hs));
         return OF_OK;
     });
 }
