@@ -37,6 +37,30 @@ def test_hypot_matches_glibc():
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def test_hypot_at_float_rounding_boundaries():
+    """Pairs whose hypot is exactly a midpoint between two floats (Pythagorean
+    triples with an odd 25-bit hypotenuse, scaled by powers of two) and the
+    same pairs moved by one ulp: where rounding the double square root to
+    float must break the tie to even, as glibc's hypotf does."""
+    rng = np.random.default_rng(7)
+    u = rng.integers(2900, 4097, 200000)
+    v = rng.integers(1, 2900, 200000)
+    A = u * u - v * v
+    B = 2 * u * v
+    M = u * u + v * v
+    keep = (A > 0) & (A < 2 ** 24) & (B < 2 ** 24) & (M >= 2 ** 24) & (M < 2 ** 25) & (M % 2 == 1)
+    A, B = A[keep], B[keep]
+    k = rng.integers(-60, 61, A.size).astype(np.float64)
+    a = (A * 2.0 ** k).astype(np.float32)
+    b = (B * 2.0 ** k).astype(np.float32)
+    a = np.concatenate([a, np.nextafter(a, np.float32(np.inf)), np.nextafter(a, np.float32(0)), b])
+    b = np.concatenate([b, b, b, a[:b.size]])
+    assert a.size > 20000
+    ref = np.hypot(a, b)
+    got = D.hypot(a, b)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("h,w", [(37, 53), (120, 68), (135, 240)])
 @pytest.mark.parametrize("sigma", [float(np.float32(0.8)), float(np.float32(1.0)),
                                    float(np.float32(0.6) * np.sqrt(3.0))])
