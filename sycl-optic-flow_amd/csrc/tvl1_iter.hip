@@ -91,9 +91,9 @@ struct TileShared {
 // registers or, across waves, from one LDS row.  Values within K pixels of
 // the tile edge go stale one pixel per iteration; the central (64-2K) x
 // (4R-2K) pixels are exact.
-template <int K, int R, bool COH>
-__device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, int cur, int kk,
-                                             double (&acc)[K], TileShared& sh)
+template <int K, int R, bool COH, bool INNER>
+__device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int by, int cur, int kk,
+                                                  double (&acc)[K], TileShared& sh)
 {
     constexpr int NW = 4;
     constexpr int TW = 64 - 2 * K;
@@ -104,12 +104,14 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
     const int x = bx * TW - K + lane;
     const int ybase = by * TH - K + wave * R;
     const int oy0 = by * TH;
-    const bool xin = x >= 0 && x < nx;
+    // INNER: the extended tile lies within columns 1..nx-2 and rows 1..ny-2,
+    // so every border test below is known and folds away
+    const bool xin = INNER || (x >= 0 && x < nx);
     const bool xout = xin && lane >= K && lane < K + TW;
-    const bool fc = x == 0, lc = x == nx - 1;
+    const bool fc = !INNER && x == 0, lc = !INNER && x == nx - 1;
     // waves holding no first/last image column take the body formula of the
     // divergence on every row away from the first/last image row
-    const bool edge_cols = __any(fc || lc);
+    const bool edge_cols = !INNER && __any(fc || lc);
 
     const float* uin = a.u[cur];
     const float* pin = a.p[cur];
@@ -119,7 +121,7 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int y = ybase + r;
-        if (xin && y >= 0 && y < ny) {
+        if (INNER || (xin && y >= 0 && y < ny)) {
             const size_t idx = (size_t)y * nx + x;
             u1[r] = ld<COH>(uin + idx);
             u2[r] = ld<COH>(uin + plane + idx);
@@ -153,7 +155,7 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
                 const float p21l = from_left(p21[r]);
                 const float p12u = r ? p12[r - 1] : up12;
                 const float p22u = r ? p22[r - 1] : up22;
-                const bool fr = y == 0, lr = y == ny - 1;
+                const bool fr = !INNER && y == 0, lr = !INNER && y == ny - 1;
                 float div1, div2;
                 if (!edge_cols && !fr && !lr) {   // uniform branch: interior rows of interior waves
                     div1 = (p11[r] - p11l) + (p12[r] - p12u);
@@ -168,7 +170,7 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
                 const float nu2 = v2 + theta * div2;
                 const float e1 = nu1 - u1[r], e2 = nu2 - u2[r];
                 const float e = e1 * e1 + e2 * e2;
-                const bool mine = xout && y >= oy0 && y < oy0 + TH && y < ny;
+                const bool mine = xout && y >= oy0 && y < oy0 + TH && (INNER || y < ny);
                 acc[t] += mine ? (double)e : 0.0;
                 u1[r] = nu1;
                 u2[r] = nu2;
@@ -186,7 +188,7 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
                 const float u2r = from_right(u2[r]);
                 const float u1d = r < R - 1 ? u1[r + 1] : dn1;
                 const float u2d = r < R - 1 ? u2[r + 1] : dn2;
-                const bool xl = x < nx - 1, yl = y < ny - 1;
+                const bool xl = INNER || x < nx - 1, yl = INNER || y < ny - 1;
                 const float fx1 = xl ? u1r - u1[r] : 0.0f;
                 const float fy1 = yl ? u1d - u1[r] : 0.0f;
                 const float fx2 = xl ? u2r - u2[r] : 0.0f;
@@ -208,7 +210,7 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int y = ybase + r;
-            if (y >= oy0 && y < oy0 + TH && y < ny) {
+            if (y >= oy0 && y < oy0 + TH && (INNER || y < ny)) {
                 const size_t idx = (size_t)y * nx + x;
                 st<COH>(uout + idx, u1[r]);
                 st<COH>(uout + plane + idx, u2[r]);
@@ -219,6 +221,20 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
             }
         }
     }
+}
+
+// Tiles whose extended area (64 columns x 4R rows from (bx*TW-K, by*TH-K))
+// stays clear of the image border run the INNER variant.
+template <int K, int R, bool COH>
+__device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, int cur, int kk,
+                                             double (&acc)[K], TileShared& sh)
+{
+    constexpr int TW = 64 - 2 * K;
+    constexpr int TH = 4 * R - 2 * K;
+    const int x0 = bx * TW - K, y0 = by * TH - K;
+    const bool inner = x0 >= 1 && x0 + 63 <= a.nx - 2 && y0 >= 1 && y0 + 4 * R - 1 <= a.ny - 2;
+    if (inner) tile_iterate_impl<K, R, COH, true>(a, bx, by, cur, kk, acc, sh);
+    else tile_iterate_impl<K, R, COH, false>(a, bx, by, cur, kk, acc, sh);
 }
 
 // Workgroup sum of the per-lane partials, fixed order: wave tree, then
