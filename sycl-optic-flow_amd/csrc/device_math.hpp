@@ -131,4 +131,29 @@ __device__ __forceinline__ float hypot_ref(float a, float b)
     return (float)__builtin_sqrt((double)a * (double)a + (double)b * (double)b);
 }
 
+// Correctly rounded double square root of s = 0 or s in [2^-767, 2^1023]:
+// the hardware reciprocal square root refined by a Goldschmidt step and two
+// Newton corrections with fused multiply-adds, the same iteration the
+// compiler emits for sqrt (which first scales s below 2^-767 by 2^256 and
+// passes 0 and infinity through, two steps that cannot apply here).
+__device__ __forceinline__ double sqrt_unscaled(double s)
+{
+    const double r = __builtin_amdgcn_rsq(s);
+    double g = s * r;
+    double h = 0.5 * r;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    g = __builtin_fma(__builtin_fma(-g, g, s), h, g);
+    g = __builtin_fma(__builtin_fma(-g, g, s), h, g);
+    return s == 0.0 ? 0.0 : g;
+}
+
+// hypot_ref for finite a, b: their double sum of squares is 0 or lies in
+// [2^-298, 2^257], inside sqrt_unscaled's range.
+__device__ __forceinline__ float hypot_fin(float a, float b)
+{
+    return (float)sqrt_unscaled((double)a * (double)a + (double)b * (double)b);
+}
+
 }  // namespace amdflow

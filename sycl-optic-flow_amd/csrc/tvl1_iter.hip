@@ -193,8 +193,8 @@ __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int
                 const float fy1 = yl ? u1d - u1[r] : 0.0f;
                 const float fx2 = xl ? u2r - u2[r] : 0.0f;
                 const float fy2 = yl ? u2d - u2[r] : 0.0f;
-                const float g1 = 1.0f + taut * hypot_ref(fx1, fy1);
-                const float g2 = 1.0f + taut * hypot_ref(fx2, fy2);
+                const float g1 = 1.0f + taut * hypot_fin(fx1, fy1);
+                const float g2 = 1.0f + taut * hypot_fin(fx2, fy2);
                 // cblas_saxpy (p += taut*grad) then the division, tvl1.cpp:322-333
                 p11[r] = (p11[r] + taut * fx1) / g1;
                 p12[r] = (p12[r] + taut * fy1) / g1;
@@ -286,6 +286,12 @@ __device__ __forceinline__ int stop_after(const double* tot, int kk, int n0, con
     return 0;
 }
 
+// Workgroups of 256 per CU the iteration kernels are compiled for: 3, i.e.
+// 3 waves per SIMD, which caps them at 168 VGPRs (allocated in steps of 8
+// out of 512).  Without the cap the largest shapes took 170 registers and
+// fell to 2 waves per SIMD, and the 1920x1080 warping slowed by 10%.
+constexpr int kIterBlocksPerCU = 3;
+
 // ---------------------------------------------------------------------------
 // launch-per-chunk driver
 // ---------------------------------------------------------------------------
@@ -295,7 +301,7 @@ __device__ __forceinline__ int stop_after(const double* tot, int kk, int n0, con
 // iteration, the launch's output is discarded and the next launch re-runs
 // exactly that many iterations from the same input buffer.
 template <int K, int R>
-__global__ __launch_bounds__(256) void k_tvl1_iter(IterArgs a)
+__global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_iter(IterArgs a)
 {
     __shared__ TileShared sh;
     __shared__ double sh_err[4][K];
@@ -434,7 +440,7 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
 // other XCDs are seen without cache maintenance.  Partials are
 // double-buffered by chunk parity.
 template <int K, int R>
-__global__ __launch_bounds__(256) void k_tvl1_persist(IterArgs a, unsigned tiles_x, unsigned ntiles)
+__global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_persist(IterArgs a, unsigned tiles_x, unsigned ntiles)
 {
     __shared__ TileShared sh;
     __shared__ double sh_err[4][K];
@@ -595,7 +601,7 @@ __device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsi
 }
 
 template <int K, int R>
-__global__ __launch_bounds__(256) void k_ls_iter(LsPhase ph, const LsPair* __restrict__ pairs,
+__global__ __launch_bounds__(256, kIterBlocksPerCU) void k_ls_iter(LsPhase ph, const LsPair* __restrict__ pairs,
                                                  LsCtl* mc, unsigned long long* host_status,
                                                  unsigned epoch)
 {

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_hypot(const float* __restrict__ a,
                                                float* __restrict__ o, unsigned n)
 {
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        o[i] = hypot_ref(a[i], b[i]);
+        o[i] = hypot_fin(a[i], b[i]);
 }
 
 // TV-L1 energy of a flow at one scale (no reference counterpart; it decides
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_energy(const float* __restrict__ I0,
         const float ay = y < ny - 1 ? u1[i + nx] - a : 0.0f;
         const float bx = x < nx - 1 ? u2[i + 1] - b : 0.0f;
         const float by = y < ny - 1 ? u2[i + nx] - b : 0.0f;
-        tv += (double)hypot_ref(ax, ay) + (double)hypot_ref(bx, by);
+        tv += (double)hypot_fin(ax, ay) + (double)hypot_fin(bx, by);
     }
     sd[threadIdx.x] = data;
     st[threadIdx.x] = tv;
