@@ -599,6 +599,20 @@ int run_lockstep(of_tvl1_batch* b, int npairs, const int* widths, const int* hei
         load[l] += (long long)widths[i] * heights[i];
     }
     for (auto& m : mine) std::sort(m.begin(), m.end());
+    // If a step fails (or throws), kernels already queued on other lanes
+    // still write into the caller's buffers and this batch's tables: wait for
+    // every lane before returning.
+    struct Drain {
+        of_tvl1_batch* b;
+        int nl;
+        bool armed = true;
+        ~Drain()
+        {
+            if (!armed) return;
+            for (int l = 0; l < nl; l++) (void)hipStreamSynchronize(b->lanes[l].stream);
+            (void)hipGetLastError();
+        }
+    } drain{b, nl};
     // rounds of at most kLsMaxPairs pairs per lane, all lanes advanced in turn
     for (size_t g = 0;; g += kLsMaxPairsPerGroup) {
         bool any = false;
@@ -635,6 +649,7 @@ int run_lockstep(of_tvl1_batch* b, int npairs, const int* widths, const int* hei
                 }
         }
     }
+    drain.armed = false;   // every lane was synchronised above
     return OF_OK;
 }
 

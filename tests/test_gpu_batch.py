@@ -90,3 +90,22 @@ def test_lockstep_launch_shapes_agree(K, R):
         uo, ito = O.tvl1(q0, q1)
         assert it[k] == int(ito.sum()), k
         assert np.array_equal(flows[k], uo), k
+
+
+@pytest.mark.parametrize("mode", ["lockstep", "workers"])
+def test_batch_rejects_unusable_pair_and_stays_usable(mode):
+    """A pair too small for the pre-smoothing Gaussian fails the call with an
+    error (no launch is left behind), and the same batch object then solves
+    a valid batch exactly."""
+    from amdflow import OpticalFlowError
+    from amdflow.batch import TVL1Batch
+    good = [(160, 120), (97, 61)]
+    frames, d0, d1 = _frames(good + [(4, 4)], 3.0, 700)
+    with TVL1Batch(streams=2, mode=mode) as b:
+        with pytest.raises(OpticalFlowError):
+            _run(b, good + [(4, 4)], d0, d1)
+        it, flows = _run(b, good, d0[:2], d1[:2])
+    for k, (q0, q1) in enumerate(frames[:2]):
+        uo, ito = O.tvl1(q0, q1)
+        assert it[k] == int(ito.sum()), k
+        assert np.array_equal(flows[k], uo), k
