@@ -19,6 +19,7 @@ missing or cannot be loaded, importing this module raises.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import os
 
 import numpy as np
@@ -150,6 +151,16 @@ def build_info() -> str:
     return _lib.of_build_info().decode()
 
 
+def _exiting() -> bool:
+    """True while the interpreter shuts down: module globals (the library
+    handle among them) may already be gone and the process releases the
+    device memory anyway, so finalisers skip their destroy calls."""
+    try:
+        return sys.is_finalizing()
+    except Exception:   # sys itself already torn down
+        return True
+
+
 def _check(status: int, what: str) -> None:
     if status != 0:
         raise OpticalFlowError(status, what)
@@ -196,7 +207,11 @@ class TV_L1:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:   # at interpreter exit this module's globals may be gone already
+            if not _exiting():
+                self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

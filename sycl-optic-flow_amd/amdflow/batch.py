@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import TVL1Params, _check, _lib
+from . import TVL1Params, _check, _lib, _exiting
 
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
@@ -67,7 +67,11 @@ class TVL1Batch:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:   # at interpreter exit this module's globals may be gone already
+            if not _exiting():
+                self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from . import _check, _lib
+from . import _check, _lib, _exiting
 
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
@@ -66,7 +66,11 @@ class WarmStartNet:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:   # at interpreter exit this module's globals may be gone already
+            if not _exiting():
+                self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

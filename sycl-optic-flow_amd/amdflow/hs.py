@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _check, _lib
+from . import _check, _lib, _exiting
 
 _vp = C.c_void_p
 _lib.of_hs_set_jacobi_shape.restype = C.c_int
@@ -55,7 +55,11 @@ class HornSchunck:
     close = deleteFlow_mem
 
     def __del__(self):
-        self.deleteFlow_mem()
+        try:   # at interpreter exit this module's globals may be gone already
+            if not _exiting():
+                self.deleteFlow_mem()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

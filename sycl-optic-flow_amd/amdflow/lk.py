@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import _check, _lib
+from . import _check, _lib, _exiting
 
 
 class LucasKanade:
@@ -39,7 +39,11 @@ class LucasKanade:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:   # at interpreter exit this module's globals may be gone already
+            if not _exiting():
+                self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
