@@ -201,10 +201,25 @@ int of_tvl1_batch_run_u8(of_tvl1_batch *b, int npairs, const int *widths, const 
                          const uint8_t *const *dI0, const uint8_t *const *dI1, float *const *dU,
                          int *iterations_total);
 
+/* One call for a batch (no reference counterpart): of_tvl1_batch_create
+ * with 3 lanes, of_tvl1_batch_run_u8, of_tvl1_batch_destroy.  Callers that
+ * solve batches repeatedly keep a batch object instead (it pools handles). */
+int of_tvl1_run_batch(const of_tvl1_params *params, int npairs, const int *widths,
+                      const int *heights, const uint8_t *const *dI0, const uint8_t *const *dI1,
+                      float *const *dU, int *iterations_total);
+
 /* ------------------------------------------------------------------------
  * Two-phase TV-L1 solve (no reference counterpart; used by warm starts).
  * of_tvl1_run_device_u8 = prepare + solve_prepared(NULL, 0, 0).
  * ---------------------------------------------------------------------- */
+/* Initial flow for the next solve of this handle (any of_tvl1_run_* or
+ * of_tvl1_push_f32 call; no reference counterpart: the reference starts the
+ * coarsest scale from zero, tvl1.cpp:199-200).  u and v (host or device
+ * memory) hold scale `level`'s flow components, of_tvl1_scale_size(level)
+ * floats each, in that scale's pixel units.  The next solve starts at
+ * `level` from that flow with the handle's warpings and iteration cap and
+ * skips the coarser scales; the flow is used once.  u = v = NULL clears it. */
+int of_tvl1_set_init_flow(of_tvl1 *h, int level, const float *u, const float *v);
 /* Phase 1 of runDualTVL1Multiscale (src/tvl1/sequential/tvl1.cpp:139-155):
  * frames to float, joint normalisation, pre-smoothing, both pyramids. */
 int of_tvl1_prepare_device_u8(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, void *stream);

@@ -116,6 +116,7 @@ SIGNATURES = {
     "of_tvl1_prepare_device_u8": (C.c_int, [_vp, _vp, _vp, _vp]),
     "of_tvl1_scale_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "of_tvl1_solve_prepared": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "of_tvl1_set_init_flow": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "of_tvl1_lift_flow": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, _vp]),
     "of_tvl1_flow_to_scale": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
     "of_tvl1_scale_flow": (C.c_int, [_vp, C.c_int, _vp, _vp]),
@@ -320,6 +321,22 @@ class TV_L1:
         a, b = _vp(), _vp()
         _check(_lib.of_tvl1_scale_images(self._h, int(s), C.byref(a), C.byref(b)), "scale_images")
         return a.value, b.value
+
+    def set_init_flow(self, level: int, u, v) -> None:
+        """Initial flow of the next solve at scale `level`: numpy arrays (host)
+        or integer device pointers, that scale's size each; None, None clears."""
+        if u is None and v is None:
+            _check(_lib.of_tvl1_set_init_flow(self._h, int(level), None, None), "set_init_flow")
+            return
+        keep = []
+
+        def ptr(x):
+            if isinstance(x, int):
+                return x
+            a = np.ascontiguousarray(x, np.float32)
+            keep.append(a)
+            return a.ctypes.data
+        _check(_lib.of_tvl1_set_init_flow(self._h, int(level), ptr(u), ptr(v)), "set_init_flow")
 
     def solve_prepared(self, d_U: int, d_Uinit: int = 0, first_scale: int = -1, warps: int = 0,
                        max_iterations: int = 0, stream: int = 0) -> None:

@@ -23,7 +23,26 @@ _lib.of_tvl1_batch_run_u8.argtypes = [_vp, C.c_int, _ip, _ip, C.POINTER(_vp), C.
                                       C.POINTER(_vp), _ip]
 
 
-MODES = {"lockstep": 0, "workers": 1}   # OF_BATCH_LOCKSTEP, OF_BATCH_WORKERS
+_lib.of_tvl1_run_batch.restype = C.c_int
+_lib.of_tvl1_run_batch.argtypes = [C.POINTER(TVL1Params), C.c_int, _ip, _ip, C.POINTER(_vp),
+                                   C.POINTER(_vp), C.POINTER(_vp), _ip]
+
+MODES = {"lockstep": 0, "workers": 1}
+
+
+def run_batch(widths, heights, d_I0, d_I1, d_U, tau=0.25, lambda_=0.15, theta=0.3,
+              nscales=100, zfactor=0.5, warps=5, epsilon=0.01, max_iterations=300):
+    """One-call batch (of_tvl1_run_batch): device frames and flows as integer
+    pointers; returns each pair's total iteration count."""
+    n = len(widths)
+    p = TVL1Params(tau, lambda_, theta, nscales, zfactor, warps, epsilon, max_iterations)
+    w = np.ascontiguousarray(widths, np.int32)
+    h = np.ascontiguousarray(heights, np.int32)
+    it = np.zeros(n, np.int32)
+    _check(_lib.of_tvl1_run_batch(C.byref(p), n, w.ctypes.data_as(_ip), h.ctypes.data_as(_ip),
+                                  (_vp * n)(*d_I0), (_vp * n)(*d_I1), (_vp * n)(*d_U),
+                                  it.ctypes.data_as(_ip)), "run_batch")
+    return it   # OF_BATCH_LOCKSTEP, OF_BATCH_WORKERS
 
 
 class TVL1Batch:

@@ -720,6 +720,17 @@ int of_tvl1_batch_set_mode(of_tvl1_batch* b, int mode)
     return OF_OK;
 }
 
+int of_tvl1_run_batch(const of_tvl1_params* params, int npairs, const int* widths,
+                      const int* heights, const uint8_t* const* dI0, const uint8_t* const* dI1,
+                      float* const* dU, int* iterations_total)
+{
+    of_tvl1_batch* b = nullptr;
+    int st = of_tvl1_batch_create(params, 3, &b);
+    if (!st) st = of_tvl1_batch_run_u8(b, npairs, widths, heights, dI0, dI1, dU, iterations_total);
+    of_tvl1_batch_destroy(b);
+    return st;
+}
+
 int of_tvl1_batch_run_u8(of_tvl1_batch* b, int npairs, const int* widths, const int* heights,
                          const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
                          int* iterations_total)

@@ -88,6 +88,7 @@ void release(of_tvl1* h)
     dev_free(h->inf);
     dev_free(h->prev);
     dev_free(h->dflow);
+    dev_free(h->init);
     dev_free(h->ctl);
     dev_free(h->partials);
     dev_free(h->energy);
@@ -364,6 +365,11 @@ int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_it
 int solve(of_tvl1* h, const void* src0, const void* src1, bool is_u8, float* dst, hipStream_t st)
 {
     OF_TRY(tvl1_prepare(h, src0, src1, is_u8, st));
+    if (h->init_level >= 0) {   // one-shot initial flow of of_tvl1_set_init_flow
+        const int first = h->init_level;
+        h->init_level = -1;
+        return solve_levels(h, h->init, first, h->prm.warps, h->prm.max_iterations, dst, st);
+    }
     return solve_levels(h, nullptr, h->ns - 1, h->prm.warps, h->prm.max_iterations, dst, st);
 }
 
@@ -606,6 +612,25 @@ int of_tvl1_scale_images(const of_tvl1* h, int s, const float** dI0s, const floa
     *dI0s = h->I0p + h->off[s];
     *dI1s = h->I1p + h->off[s];
     return OF_OK;
+}
+
+int of_tvl1_set_init_flow(of_tvl1* h, int level, const float* u, const float* v)
+{
+    return guarded([&]() -> int {
+        if (!h) return OF_ERR_ARG;
+        if (!u && !v) {
+            h->init_level = -1;
+            return OF_OK;
+        }
+        if (!u || !v || level < 0 || level >= h->ns) return OF_ERR_ARG;
+        if (!h->init) OF_TRY(dev_alloc(&h->init, 2 * (size_t)h->W * h->H));
+        const size_t n = (size_t)h->nx[level] * h->ny[level];
+        // host or device pointers alike (one address space); synchronous
+        OF_HIP(hipMemcpy(h->init, u, sizeof(float) * n, hipMemcpyDefault));
+        OF_HIP(hipMemcpy(h->init + n, v, sizeof(float) * n, hipMemcpyDefault));
+        h->init_level = level;
+        return OF_OK;
+    });
 }
 
 int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int first_scale, int warps,

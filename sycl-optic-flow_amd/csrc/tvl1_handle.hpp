@@ -53,6 +53,8 @@ struct of_tvl1 {
     float* prev = nullptr;      // previous frame of the streaming API
     bool have_prev = false;
     float* dflow = nullptr;     // 2*N0
+    float* init = nullptr;      // 2*N0, allocated on first use: of_tvl1_set_init_flow's flow
+    int init_level = -1;        // scale the next solve starts from with `init`; -1: none
     float* hflow = nullptr;     // pinned host copy returned by of_tvl1_flow
     amdflow::TvCtl* ctl = nullptr;
     double* partials = nullptr;

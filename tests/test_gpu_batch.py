@@ -109,3 +109,18 @@ def test_batch_rejects_unusable_pair_and_stays_usable(mode):
         uo, ito = O.tvl1(q0, q1)
         assert it[k] == int(ito.sum()), k
         assert np.array_equal(flows[k], uo), k
+
+
+def test_run_batch_one_call_matches_batch_object():
+    import torch
+    from amdflow.batch import TVL1Batch, run_batch
+    sizes = [(160, 120), (97, 61), (320, 240)]
+    frames, d0, d1 = _frames(sizes, 3.0, 900)
+    with TVL1Batch(streams=3) as b:
+        it, flows = _run(b, sizes, d0, d1)
+    dU = [torch.empty(2 * w * h, dtype=torch.float32, device="cuda") for (w, h) in sizes]
+    it2 = run_batch([w for w, _ in sizes], [h for _, h in sizes], [t.data_ptr() for t in d0],
+                    [t.data_ptr() for t in d1], [t.data_ptr() for t in dU])
+    assert np.array_equal(it, it2)
+    for k in range(len(sizes)):
+        assert np.array_equal(dU[k].cpu().numpy(), flows[k]), k

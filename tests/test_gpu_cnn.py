@@ -393,3 +393,39 @@ def test_trained_network_beats_zero_init_at_same_limits():
     assert res["fallbacks"] == 0, res
     assert e["warm"] < 0.5 * e["zero_init_same_limits"], res
     assert g["warm"] < g["zero"], res
+
+
+def test_set_init_flow_matches_solve_from_scale():
+    """of_tvl1_set_init_flow (host arrays) seeds the next run exactly as
+    solve_prepared seeds scale `level`, is used once, and can be cleared."""
+    torch = _torch()
+    q0, q1 = _pair(320, 240, 4.0, 31)
+    h, w = q0.shape
+    tv = TV_L1(w, h, nscales=4)
+    d0 = torch.from_numpy(q0.reshape(-1)).cuda()
+    d1 = torch.from_numpy(q1.reshape(-1)).cuda()
+    plain = torch.empty(2 * w * h, dtype=torch.float32, device="cuda")
+    tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), plain.data_ptr())
+    for level in (1, 2):
+        nx, ny = tv.scale_size(level)
+        seed = torch.empty(2 * nx * ny, dtype=torch.float32, device="cuda")
+        tv.flow_to_scale(plain.data_ptr(), level, seed.data_ptr())
+        torch.cuda.synchronize()
+        ref = torch.empty_like(plain)
+        tv.prepare_device_u8(d0.data_ptr(), d1.data_ptr())
+        tv.solve_prepared(ref.data_ptr(), seed.data_ptr(), level)
+        s = seed.cpu().numpy()
+        tv.set_init_flow(level, s[:nx * ny], s[nx * ny:])
+        got = torch.empty_like(plain)
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), got.data_ptr())
+        assert torch.equal(got, ref), level
+        again = torch.empty_like(plain)                      # used once
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), again.data_ptr())
+        assert torch.equal(again, plain), level
+    nx, ny = tv.scale_size(1)
+    tv.set_init_flow(1, np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32))
+    tv.set_init_flow(1, None, None)                          # cleared
+    cleared = torch.empty_like(plain)
+    tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), cleared.data_ptr())
+    assert torch.equal(cleared, plain)
+    tv.close()
