@@ -265,24 +265,51 @@ def warm_start(npairs: int):
     return res
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(q0, q1, budget_s: float):
-    """Times the oracle on the same pair; returns (record, flow, iterations)."""
+    """Times the oracle on the same pair, as the reference is timed
+    (runDualTVL1Multiscale only, construction excluded): one warm-up solve,
+    then the median of the solves that fit in `budget_s` (at least 5) with
+    OpenMP on all usable cores, and the median of 2 single-threaded solves.
+    Returns (record, flow, iterations)."""
+    import statistics
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # CPU checker, used here only as the measured baseline
 
     p = O.default_params(nscales=5)
-    t0 = time.perf_counter()
-    runs = 0
-    u = it = None
-    while True:
-        u, it = O.tvl1(q0, q1, p)
-        runs += 1
-        if time.perf_counter() - t0 >= budget_s or runs >= 50:
-            break
-    dt = time.perf_counter() - t0
     cores = _cores()
-    rec = {"value": runs / dt, "unit": "frame-pairs/s", "cores": cores, "kind": "port",
-           "sample": f"{runs} solve(s) of the same 1920x1080 u8 pair, OpenMP, {dt:.1f} s"}
+
+    def sample(threads, min_runs, budget):
+        O.set_threads(threads)
+        u, it = O.tvl1(q0, q1, p)                     # warm-up
+        times = []
+        t0 = time.perf_counter()
+        while len(times) < min_runs or (time.perf_counter() - t0 < budget and len(times) < 50):
+            t1 = time.perf_counter()
+            u, it = O.tvl1(q0, q1, p)
+            times.append(time.perf_counter() - t1)
+        return statistics.median(times), len(times), u, it
+
+    med, runs, u, it = sample(cores, 5, budget_s)
+    med1, runs1, _, _ = sample(1, 2, 0.0)
+    O.set_threads(cores)
+    rec = {"value": 1.0 / med, "unit": "frame-pairs/s", "cores": cores, "kind": "port",
+           "sample": f"median of {runs} solves of the same 1920x1080 u8 pair after a warm-up, "
+                     f"OpenMP {cores} threads",
+           "single_thread": {"value": round(1.0 / med1, 4), "unit": "frame-pairs/s",
+                             "sample": f"median of {runs1} solves, 1 thread"},
+           "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
     return rec, u, it
 
 

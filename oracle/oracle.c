@@ -44,6 +44,9 @@
 #include <string.h>
 
 #include "oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* tvl1.hpp:6-8, zoom.hpp:4, mask.hpp:7 */
 #define ORC_PRESMOOTHING_SIGMA 0.8f
@@ -514,6 +517,26 @@ int orc_dual_tvl1(const float *I0, const float *I1, float *u1, float *u2, int nx
     }
     free(buf);
     return ORC_OK;
+}
+
+/* OpenMP threads of the oracle's parallel loops (the reference runs with all
+ * cores, OMP_NUM_THREADS); the results do not depend on the count. */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 void orc_tvl1_default_params(orc_tvl1_params *p)

@@ -22,6 +22,9 @@ typedef struct {
     int max_iterations;
 } orc_tvl1_params;
 
+/* OpenMP threads of the parallel loops (results do not depend on it). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 void orc_tvl1_default_params(orc_tvl1_params *p);
 int orc_tvl1_nscales(int width, int height, int nscales, float zfactor);
 void orc_zoom_size(int nx, int ny, float factor, int *nxx, int *nyy);

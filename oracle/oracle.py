@@ -50,6 +50,17 @@ def _sig(name, res, args):
 
 
 _sig("orc_tvl1_default_params", None, [C.POINTER(Params)])
+_sig("orc_set_threads", None, [C.c_int])
+_sig("orc_get_threads", C.c_int, [])
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's loops (its results do not depend on it)."""
+    _lib.orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(_lib.orc_get_threads())
 _sig("orc_tvl1_nscales", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float])
 _sig("orc_zoom_size", None, [C.c_int, C.c_int, C.c_float, _ip, _ip])
 _sig("orc_zoom_sigma", C.c_float, [C.c_float])
