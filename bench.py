@@ -50,6 +50,9 @@ def parse():
                     help="pairs of the batched mixed-size measurement (0 = skip)")
     ap.add_argument("--batch-lanes", type=int, default=3,
                     help="lanes (concurrent streams) of the lockstep batch solver")
+    ap.add_argument("--batch-reps", type=int, default=3, help="timed runs of the batch")
+    ap.add_argument("--no-batch-workers", action="store_true",
+                    help="skip the worker-mode comparison of the batch")
     ap.add_argument("--no-other", action="store_true", help="skip the HS and LK measurements")
     ap.add_argument("--warm-pairs", type=int, default=16,
                     help="pairs of the warm-start measurement (0 = skip)")
@@ -77,7 +80,7 @@ def calibrate_copy():
 BATCH_SIZES = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
 
 
-def batched(npairs: int, lanes: int, reps: int):
+def batched(npairs: int, lanes: int, reps: int, workers: bool = True):
     """64 pairs whose sizes are drawn uniformly (seeded) from BATCH_SIZES,
     same generator and parameters as the 1920x1080 workload, solved by the
     lockstep batch solver (and, for comparison, its worker mode); two
@@ -125,7 +128,7 @@ def batched(npairs: int, lanes: int, reps: int):
             return (time.perf_counter() - t0) / reps, its
 
     dt, its = timed("lockstep", lanes)
-    dt_w, its_w = timed("workers", 4)
+    dt_w, its_w = timed("workers", 4) if workers else (None, its)
     same = all(int(its[i]) == alg[k][1] for i, k in enumerate(keys)) and bool(np.array_equal(its, its_w))
     B = sum(alg[k][0] for k in keys)
     mpx = sum(w * h for w, h in zip(ws, hs)) / 1e6
@@ -136,8 +139,9 @@ def batched(npairs: int, lanes: int, reps: int):
             "mpixel_per_s": round(mpx / dt, 1), "iterations_total": int(np.sum(its)),
             "algorithmic_bytes": B, "algorithmic_GBps": round(B / dt / 1e9, 1),
             "hbm_roofline_frac": round(B / dt / 8e12, 4),
-            "workers_mode": {"streams": 4, "ms_per_batch": round(dt_w * 1e3, 2),
-                             "hbm_roofline_frac": round(B / dt_w / 8e12, 4)},
+            "workers_mode": None if dt_w is None else {
+                "streams": 4, "ms_per_batch": round(dt_w * 1e3, 2),
+                "hbm_roofline_frac": round(B / dt_w / 8e12, 4)},
             "iterations_match_single_solves": same,
             "data": "synthetic, 2 generated pairs per size cycled through the batch"}
 
@@ -421,7 +425,8 @@ def main():
         "roofline": roofline,
     }
     if args.batch_pairs > 0:
-        result["batched"] = batched(args.batch_pairs, args.batch_lanes, reps=3)
+        result["batched"] = batched(args.batch_pairs, args.batch_lanes, reps=args.batch_reps,
+                                    workers=not args.no_batch_workers)
     if args.warm_pairs > 0:
         result["warm_start"] = warm_start(args.warm_pairs)
     if not args.no_other:
