@@ -173,6 +173,94 @@ __global__ __launch_bounds__(256) void k_lk_tensor_products(const float* __restr
     t3[2 * n + p] = gy * gy;
 }
 
+// The gradient, its products and their window sums in one launch, on a
+// 64 x 16 output tile: the products on the tile plus a WC-pixel halo (LDS),
+// their row sums, then the column sums into G (Gxx, Gxy, Gyy planes); Ix and
+// Iy are written for the output pixels.  Same expressions and summation
+// order as k_lk_tensor_products followed by k_lk_boxsum along rows and then
+// columns (windows clipped to the image, ascending order).
+template <int WC>
+__global__ __launch_bounds__(256) void k_lk_tensor_fused(const float* __restrict__ P0, int w,
+                                                         int h, float* __restrict__ Ix,
+                                                         float* __restrict__ Iy,
+                                                         float* __restrict__ G)
+{
+    constexpr int TW = 64, TH = 16, EW = TW + 2 * WC, EH = TH + 2 * WC;
+    constexpr int NE = (EW * EH + 255) / 256;
+    constexpr int NO = TW * TH / 256;
+    __shared__ float pr[3][EH][EW];
+    __shared__ float rs[3][EH][TW];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const size_t n = (size_t)w * h;
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = threadIdx.x + 256 * k;
+        if (e >= EW * EH) break;
+        const int ey = e / EW, ex = e - ey * EW;
+        const int j = x0 - WC + ex, i = y0 - WC + ey;
+        float a = 0.0f, b = 0.0f, c = 0.0f;
+        if (j >= 0 && j < w && i >= 0 && i < h) {
+            const int jl = j > 0 ? j - 1 : 0, jr = j < w - 1 ? j + 1 : w - 1;
+            const int iu = i > 0 ? i - 1 : 0, id = i < h - 1 ? i + 1 : h - 1;
+            const float ddx = P0[(size_t)i * w + jr] - P0[(size_t)i * w + jl];
+            const float ddy = P0[(size_t)id * w + j] - P0[(size_t)iu * w + j];
+            const float gx = (float)(0.5 * (double)ddx), gy = (float)(0.5 * (double)ddy);
+            a = gx * gx;
+            b = gx * gy;
+            c = gy * gy;
+            const int ox = ex - WC, oy = ey - WC;
+            if (ox >= 0 && ox < TW && oy >= 0 && oy < TH) {
+                const size_t p = (size_t)i * w + j;
+                Ix[p] = gx;
+                Iy[p] = gy;
+            }
+        }
+        pr[0][ey][ex] = a;
+        pr[1][ey][ex] = b;
+        pr[2][ey][ex] = c;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TW * EH; e += 256) {
+        const int ey = e / TW, ox = e - ey * TW;
+        const int gx = x0 + ox;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int k = -WC; k <= WC; k++) {
+            const int jj = gx + k;
+            if (jj >= 0 && jj < w) {
+                s0 += pr[0][ey][ox + WC + k];
+                s1 += pr[1][ey][ox + WC + k];
+                s2 += pr[2][ey][ox + WC + k];
+            }
+        }
+        rs[0][ey][ox] = s0;
+        rs[1][ey][ox] = s1;
+        rs[2][ey][ox] = s2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NO; k++) {
+        const int e = threadIdx.x + 256 * k;
+        const int oy = e / TW, ox = e - oy * TW;
+        const int gx = x0 + ox, gy = y0 + oy;
+        if (gx >= w || gy >= h) continue;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int kk = -WC; kk <= WC; kk++) {
+            const int ii = gy + kk;
+            if (ii >= 0 && ii < h) {
+                s0 += rs[0][oy + WC + kk][ox];
+                s1 += rs[1][oy + WC + kk][ox];
+                s2 += rs[2][oy + WC + kk][ox];
+            }
+        }
+        const size_t p = (size_t)gy * w + gx;
+        G[p] = s0;
+        G[n + p] = s1;
+        G[2 * n + p] = s2;
+    }
+}
+
 // Row (dir 0) or column (dir 1) window sums of `planes` planes, window
 // clipped to the image, ascending order.
 __global__ __launch_bounds__(256) void k_lk_boxsum(const float* __restrict__ in,
@@ -251,7 +339,10 @@ __global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ row
 // column sums and the 2x2 update.  Same expressions and summation order as
 // k_lk_it_products / k_lk_boxsum / k_lk_update (products outside the image
 // are never added: the windows are clipped to the image); reads (u, v),
-// writes the updated flow to (uo, vo).
+// writes the updated flow to (uo, vo).  Every global load a thread needs is
+// issued before the first use: the output pixels' flow and tensor, then the
+// extended tile's flow, then its samples, so one launch waits on two memory
+// round trips, not one per element.
 template <int WC>
 __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__ P0,
                                                        const float* __restrict__ P1,
@@ -264,17 +355,53 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
                                                        float* __restrict__ vo)
 {
     constexpr int TW = 64, TH = 16, EW = TW + 2 * WC, EH = TH + 2 * WC;
+    constexpr int NE = (EW * EH + 255) / 256;   // extended-tile elements per thread
+    constexpr int NO = TW * TH / 256;           // output pixels per thread
     __shared__ float pr[2][EH][EW];
     __shared__ float rs[2][EH][TW];
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const size_t n = (size_t)w * h;
-    for (int e = threadIdx.x; e < EW * EH; e += 256) {
+
+    float ou[NO], ov[NO], gxx[NO], gxy[NO], gyy[NO];
+#pragma unroll
+    for (int k = 0; k < NO; k++) {
+        const int e = threadIdx.x + 256 * k;
+        const int oy = e / TW, ox = e - oy * TW;
+        const int gx = x0 + ox, gy = y0 + oy;
+        ou[k] = ov[k] = gxx[k] = gxy[k] = gyy[k] = 0.0f;
+        if (gx < w && gy < h) {
+            const size_t p = (size_t)gy * w + gx;
+            ou[k] = u[p];
+            ov[k] = v[p];
+            gxx[k] = G[p];
+            gxy[k] = G[n + p];
+            gyy[k] = G[2 * n + p];
+        }
+    }
+
+    float eu[NE], ev[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = threadIdx.x + 256 * k;
+        const int ey = e / EW, ex = e - ey * EW;
+        const int gx = x0 - WC + ex, gy = y0 - WC + ey;
+        eu[k] = ev[k] = 0.0f;
+        if (e < EW * EH && gx >= 0 && gx < w && gy >= 0 && gy < h) {
+            const size_t p = (size_t)gy * w + gx;
+            eu[k] = u[p];
+            ev[k] = v[p];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = threadIdx.x + 256 * k;
+        if (e >= EW * EH) break;
         const int ey = e / EW, ex = e - ey * EW;
         const int gx = x0 - WC + ex, gy = y0 - WC + ey;
         float a = 0.0f, b = 0.0f;
         if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
             const size_t p = (size_t)gy * w + gx;
-            const float wv = bilinear_at(P1, w, h, (float)gx + u[p], (float)gy + v[p]);
+            const float wv = bilinear_at(P1, w, h, (float)gx + eu[k], (float)gy + ev[k]);
             const float itv = wv - P0[p];
             a = Ix[p] * itv;
             b = Iy[p] * itv;
@@ -299,26 +426,27 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
         rs[1][ey][ox] = s1;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < TW * TH; e += 256) {
+#pragma unroll
+    for (int k = 0; k < NO; k++) {
+        const int e = threadIdx.x + 256 * k;
         const int oy = e / TW, ox = e - oy * TW;
         const int gx = x0 + ox, gy = y0 + oy;
         if (gx >= w || gy >= h) continue;
         float bx = 0.0f, by = 0.0f;
 #pragma unroll
-        for (int k = -WC; k <= WC; k++) {
-            const int ii = gy + k;
+        for (int kk = -WC; kk <= WC; kk++) {
+            const int ii = gy + kk;
             if (ii >= 0 && ii < h) {
-                bx += rs[0][oy + WC + k][ox];
-                by += rs[1][oy + WC + k][ox];
+                bx += rs[0][oy + WC + kk][ox];
+                by += rs[1][oy + WC + kk][ox];
             }
         }
         const size_t p = (size_t)gy * w + gx;
-        const float gxx = G[p], gxy = G[n + p], gyy = G[2 * n + p];
-        const float det = gxx * gyy - gxy * gxy;
-        float nu = u[p], nv = v[p];
+        const float det = gxx[k] * gyy[k] - gxy[k] * gxy[k];
+        float nu = ou[k], nv = ov[k];
         if (det > det_eps) {
-            const float du = (gxy * by - gyy * bx) / det;
-            const float dv = (gxy * bx - gxx * by) / det;
+            const float du = (gxy[k] * by - gyy[k] * bx) / det;
+            const float dv = (gxy[k] * bx - gxx[k] * by) / det;
             nu = nu + du;
             nv = nv + dv;
         }
@@ -439,14 +567,25 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
             const int w = nx[l], h = ny[l];
             const float* p0 = P0 + off[l];
             const float* p1 = P1 + off[l];
-            hipLaunchKernelGGL(k_lk_tensor_products, grid2d(w, h), dim3(256), 0, st, p0, w, h, Ix, Iy,
-                               PROD);
-            LK_CHECK("k_lk_tensor_products");
-            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 3, wc, 0);
-            LK_CHECK("k_lk_boxsum");
-            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, 3, wc, 1);
-            LK_CHECK("k_lk_boxsum");
             const dim3 fgrid(cdiv(w, 64), cdiv(h, 16));
+            if (wc >= 1 && wc <= 4) {
+                // gradient, products and both window sums in one launch
+                if (wc == 1) hipLaunchKernelGGL((k_lk_tensor_fused<1>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+                if (wc == 2) hipLaunchKernelGGL((k_lk_tensor_fused<2>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+                if (wc == 3) hipLaunchKernelGGL((k_lk_tensor_fused<3>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+                if (wc == 4) hipLaunchKernelGGL((k_lk_tensor_fused<4>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+                LK_CHECK("k_lk_tensor_fused");
+            } else {
+                hipLaunchKernelGGL(k_lk_tensor_products, grid2d(w, h), dim3(256), 0, st, p0, w, h, Ix,
+                                   Iy, PROD);
+                LK_CHECK("k_lk_tensor_products");
+                hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 3, wc,
+                                   0);
+                LK_CHECK("k_lk_boxsum");
+                hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, 3, wc,
+                                   1);
+                LK_CHECK("k_lk_boxsum");
+            }
             for (int it = 0; it < iters && wc >= 1 && wc <= 4; it++) {
                 // the whole iteration in one kernel, (u, v) -> (nu, nv)
                 if (wc == 1) hipLaunchKernelGGL((k_lk_iter_fused<1>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
