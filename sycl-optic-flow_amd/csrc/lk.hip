@@ -22,6 +22,7 @@
 #include "amdflow_debug.h"
 #include "common.hpp"
 #include "tvl1_launch.hpp"
+#include "tvl1_pixel.hpp"
 
 namespace amdflow {
 namespace {
@@ -455,6 +456,42 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
     }
 }
 
+// One pyramid step of both frames per launch (blockIdx.z = frame): the
+// Gaussian rows, the Gaussian columns, then zoom_out's resampling, with the
+// pixel bodies of the TV-L1 pyramid (mask.cpp:217-287, zoom.cpp:28-52).
+__global__ __launch_bounds__(256) void k_lk_pyr_rows(const float* __restrict__ a,
+                                                     const float* __restrict__ b,
+                                                     float* __restrict__ oa, float* __restrict__ ob,
+                                                     int nx, int ny, GaussTaps g)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= nx || y >= ny) return;
+    (blockIdx.z ? ob : oa)[(size_t)y * nx + x] = gauss_row_px(blockIdx.z ? b : a, nx, x, y, g);
+}
+
+__global__ __launch_bounds__(256) void k_lk_pyr_cols(const float* __restrict__ a,
+                                                     const float* __restrict__ b,
+                                                     float* __restrict__ oa, float* __restrict__ ob,
+                                                     int nx, int ny, GaussTaps g)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= nx || y >= ny) return;
+    (blockIdx.z ? ob : oa)[(size_t)y * nx + x] = gauss_col_px(blockIdx.z ? b : a, nx, ny, x, y, g);
+}
+
+__global__ __launch_bounds__(256) void k_lk_pyr_zoom(const float* __restrict__ a,
+                                                     const float* __restrict__ b,
+                                                     float* __restrict__ oa, float* __restrict__ ob,
+                                                     int nx, int ny, int nxx, int nyy, float factor)
+{
+    const int j1 = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i1 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j1 >= nxx || i1 >= nyy) return;
+    (blockIdx.z ? ob : oa)[(size_t)i1 * nxx + j1] = zoom_out_px(blockIdx.z ? b : a, nx, ny, i1, j1, factor);
+}
+
 __global__ __launch_bounds__(256) void k_lk_zero2(float* __restrict__ a, float* __restrict__ b,
                                                   size_t n)
 {
@@ -548,15 +585,37 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
     float* nu = ws.nu;
     float* nv = ws.nv;
     auto run = [&]() -> int {
-        OF_HIP(hipMemcpyAsync(P0, dI0, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
-        OF_HIP(hipMemcpyAsync(P1, dI1, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        // level 0 is read from the caller's frames; levels 1.. live in P0/P1.
+        // Each pyramid step treats both frames in one launch, with PROD and
+        // ROWS (free until the structure tensor) as the two blur buffers.
+        const float* L0[16];
+        const float* L1[16];
+        for (int l = 0; l < levels; l++) {
+            L0[l] = l ? P0 + off[l] : dI0;
+            L1[l] = l ? P1 + off[l] : dI1;
+        }
         for (int l = 1; l < levels; l++) {
-            OF_TRY(launch_gaussian(P0 + off[l - 1], A0, A1, nx[l - 1], ny[l - 1], g, st));
-            OF_TRY(launch_zoom_out_sample(A1, P0 + off[l], nx[l - 1], ny[l - 1], nx[l], ny[l], zf, st));
-            OF_TRY(launch_gaussian(P1 + off[l - 1], A0, A1, nx[l - 1], ny[l - 1], g, st));
-            OF_TRY(launch_zoom_out_sample(A1, P1 + off[l], nx[l - 1], ny[l - 1], nx[l], ny[l], zf, st));
+            const int pw = nx[l - 1], ph = ny[l - 1];
+            const dim3 gb(cdiv(pw, 64), cdiv(ph, 4), 2), gz(cdiv(nx[l], 64), cdiv(ny[l], 4), 2);
+            hipLaunchKernelGGL(k_lk_pyr_rows, gb, dim3(256), 0, st, L0[l - 1], L1[l - 1], PROD,
+                               PROD + n0, pw, ph, g);
+            LK_CHECK("k_lk_pyr_rows");
+            hipLaunchKernelGGL(k_lk_pyr_cols, gb, dim3(256), 0, st, PROD, PROD + n0, ROWS, ROWS + n0,
+                               pw, ph, g);
+            LK_CHECK("k_lk_pyr_cols");
+            hipLaunchKernelGGL(k_lk_pyr_zoom, gz, dim3(256), 0, st, ROWS, ROWS + n0, P0 + off[l],
+                               P1 + off[l], pw, ph, nx[l], ny[l], zf);
+            LK_CHECK("k_lk_pyr_zoom");
         }
         const int wc = (win - 1) / 2;
+        // the last fused iteration at level 0 writes the caller's du/dv
+        // directly unless they overlap the frames or each other
+        auto overlaps = [](const float* a, const float* b, size_t len) {
+            return a < b + len && b < a + len;
+        };
+        const bool direct = wc >= 1 && wc <= 4 && iters >= 1 && !overlaps(du, dv, n0) &&
+                            !overlaps(du, dI0, n0) && !overlaps(du, dI1, n0) &&
+                            !overlaps(dv, dI0, n0) && !overlaps(dv, dI1, n0);
         const float invZ = 1 / zf;
         {
             const size_t nc = (size_t)nx[levels - 1] * ny[levels - 1];
@@ -565,8 +624,8 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
         }
         for (int l = levels - 1; l >= 0; l--) {
             const int w = nx[l], h = ny[l];
-            const float* p0 = P0 + off[l];
-            const float* p1 = P1 + off[l];
+            const float* p0 = L0[l];
+            const float* p1 = L1[l];
             const dim3 fgrid(cdiv(w, 64), cdiv(h, 16));
             if (wc >= 1 && wc <= 4) {
                 // gradient, products and both window sums in one launch
@@ -588,6 +647,10 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
             }
             for (int it = 0; it < iters && wc >= 1 && wc <= 4; it++) {
                 // the whole iteration in one kernel, (u, v) -> (nu, nv)
+                if (direct && l == 0 && it == iters - 1) {
+                    nu = du;
+                    nv = dv;
+                }
                 if (wc == 1) hipLaunchKernelGGL((k_lk_iter_fused<1>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
                 if (wc == 2) hipLaunchKernelGGL((k_lk_iter_fused<2>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
                 if (wc == 3) hipLaunchKernelGGL((k_lk_iter_fused<3>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
@@ -613,8 +676,10 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
                 t = v; v = nv; nv = t;
             }
         }
-        OF_HIP(hipMemcpyAsync(du, u, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
-        OF_HIP(hipMemcpyAsync(dv, v, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        if (!direct) {
+            OF_HIP(hipMemcpyAsync(du, u, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+            OF_HIP(hipMemcpyAsync(dv, v, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        }
         OF_HIP(hipStreamSynchronize(st));
         return OF_OK;
     };
