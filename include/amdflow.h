@@ -346,7 +346,8 @@ void of_hs_destroy(of_hs *h);
 int of_hs_compute(of_hs *h, const float *I0, const float *I1, int width, int height, int stride,
                   float alpha, int nLevels, int nWarpIters, int nSolverIters, float *u, float *v);
 /* Jacobi launch shape: `sweeps_per_launch` sweeps per launch on tiles of
- * 64 x (4*rows_per_wave) pixels (same supported set as the TV-L1 shapes). */
+ * 64 x (8*rows_per_wave) pixels (8 waves per workgroup); supported:
+ * sweeps_per_launch in {1,2,4,8}, rows_per_wave in {2,4,8}. */
 int of_hs_set_jacobi_shape(of_hs *h, int sweeps_per_launch, int rows_per_wave);
 /* Device-pointer variant (same layout), synchronous on `stream`. */
 int of_hs_compute_device(of_hs *h, const float *dI0, const float *dI1, int width, int height,
