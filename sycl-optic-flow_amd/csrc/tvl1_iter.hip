@@ -77,25 +77,27 @@ __device__ __forceinline__ void st(float* p, float v)
     else *p = v;
 }
 
-struct TileShared {
-    float a[4][64], b[4][64], c[4][64], d[4][64];
+// One LDS row per wave and exchanged quantity; NW waves per workgroup.
+template <int NW>
+struct TileSharedN {
+    float a[NW][64], b[NW][64], c[NW][64], d[NW][64];
 };
+using TileShared = TileSharedN<4>;
 
 // kk (<= K) iterations on tile (bx, by): reads buffer `cur`, writes the
 // central tile into buffer cur^1, adds each iteration's per-lane error sum
 // over the central tile to acc[t].
 //
-// One workgroup = 4 waves; the extended tile is 64 columns (one per lane) by
-// 4*R rows (R consecutive rows per wave, in registers).  Horizontal
+// One workgroup = NW waves; the extended tile is 64 columns (one per lane) by
+// NW*R rows (R consecutive rows per wave, in registers).  Horizontal
 // neighbours come from DPP lane shifts, vertical ones from the lane's own
 // registers or, across waves, from one LDS row.  Values within K pixels of
 // the tile edge go stale one pixel per iteration; the central (64-2K) x
-// (4R-2K) pixels are exact.
-template <int K, int R, bool COH, bool INNER>
+// (NW*R-2K) pixels are exact.
+template <int K, int R, bool COH, bool INNER, int NW>
 __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int by, int cur, int kk,
-                                                  double (&acc)[K], TileShared& sh)
+                                                  double (&acc)[K], TileSharedN<NW>& sh)
 {
-    constexpr int NW = 4;
     constexpr int TW = 64 - 2 * K;
     constexpr int TH = NW * R - 2 * K;
     const int lane = threadIdx.x & 63;
@@ -225,21 +227,22 @@ __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int
 
 // Tiles whose extended area (64 columns x 4R rows from (bx*TW-K, by*TH-K))
 // stays clear of the image border run the INNER variant.
-template <int K, int R, bool COH>
+template <int K, int R, bool COH, int NW>
 __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, int cur, int kk,
-                                             double (&acc)[K], TileShared& sh)
+                                             double (&acc)[K], TileSharedN<NW>& sh)
 {
     constexpr int TW = 64 - 2 * K;
-    constexpr int TH = 4 * R - 2 * K;
+    constexpr int TH = NW * R - 2 * K;
     const int x0 = bx * TW - K, y0 = by * TH - K;
-    const bool inner = x0 >= 1 && x0 + 63 <= a.nx - 2 && y0 >= 1 && y0 + 4 * R - 1 <= a.ny - 2;
-    if (inner) tile_iterate_impl<K, R, COH, true>(a, bx, by, cur, kk, acc, sh);
-    else tile_iterate_impl<K, R, COH, false>(a, bx, by, cur, kk, acc, sh);
+    const bool inner = x0 >= 1 && x0 + 63 <= a.nx - 2 && y0 >= 1 && y0 + NW * R - 1 <= a.ny - 2;
+    if (inner) tile_iterate_impl<K, R, COH, true, NW>(a, bx, by, cur, kk, acc, sh);
+    else tile_iterate_impl<K, R, COH, false, NW>(a, bx, by, cur, kk, acc, sh);
 }
 
 // Workgroup sum of the per-lane partials, fixed order: wave tree, then
-// waves 0..3.  Result valid in thread t for t < K (thread t holds sum t).
-template <int K>
+// waves 0..NW-1 left to right.  Result valid in thread t for t < K (thread t
+// holds sum t).
+template <int K, int NW = 4>
 __device__ __forceinline__ double block_sum(const double (&acc)[K], double (*sh_err)[K], int t_out)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -250,14 +253,20 @@ __device__ __forceinline__ double block_sum(const double (&acc)[K], double (*sh_
     }
     __syncthreads();
     double s = 0.0;
-    if (t_out < K) s = ((sh_err[0][t_out] + sh_err[1][t_out]) + sh_err[2][t_out]) + sh_err[3][t_out];
+    if (t_out < K) {
+        s = sh_err[0][t_out];
+#pragma unroll
+        for (int w = 1; w < NW; w++) s += sh_err[w][t_out];
+    }
     __syncthreads();
     return s;
 }
 
 // Sum over all workgroups of column t of `part` (nb entries), fixed order:
-// thread j sums entries j, j+256, ...; then wave trees and waves 0..3.
-// Identical in every workgroup that calls it on the same data.
+// thread j sums entries j, j+64*NW, ...; then wave trees and waves 0..NW-1
+// left to right.  Identical in every workgroup that calls it on the same
+// data.
+template <int NW = 4>
 __device__ __forceinline__ double grid_sum(const double* part, unsigned nb, double* sh4)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -267,7 +276,9 @@ __device__ __forceinline__ double grid_sum(const double* part, unsigned nb, doub
     s = wave_sum_f64(s);
     if (lane == 0) sh4[wave] = s;
     __syncthreads();
-    const double tot = ((sh4[0] + sh4[1]) + sh4[2]) + sh4[3];
+    double tot = sh4[0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) tot += sh4[w];
     __syncthreads();
     return tot;
 }
@@ -286,11 +297,23 @@ __device__ __forceinline__ int stop_after(const double* tot, int kk, int n0, con
     return 0;
 }
 
-// Workgroups of 256 per CU the iteration kernels are compiled for: 3, i.e.
-// 3 waves per SIMD, which caps them at 168 VGPRs (allocated in steps of 8
-// out of 512).  Without the cap the largest shapes took 170 registers and
-// fell to 2 waves per SIMD, and the 1920x1080 warping slowed by 10%.
+// Workgroups of 256 per CU the 4-wave iteration kernels are compiled for:
+// 3, i.e. 3 waves per SIMD, which caps them at 168 VGPRs (allocated in steps
+// of 8 out of 512).  Without the cap the largest shapes took 170 registers
+// and fell to 2 waves per SIMD, and the 1920x1080 warping slowed by 10%.
 constexpr int kIterBlocksPerCU = 3;
+
+// Waves per workgroup of the one-pair iteration kernels.  A launch shape
+// (K, R) names tiles of 64 x 4R pixels; they run as 8 waves of R/2 rows,
+// which hides the latency of an iteration better than 4 waves of R rows:
+// host-paced, every shape was faster at every scale of a 1920x1080 pyramid
+// (e.g. (4,4) at 480x270: 5.22 -> 4.87 us per iteration; (4,8) at 960x540:
+// 9.29 -> 8.18 us).  Compiled for 2 workgroups per CU (128 VGPRs).
+constexpr int kIterWaves = 8;
+// The lockstep batch kernel keeps 4 waves of R rows: with 8 waves of R/2 the
+// batch of 64 mixed-size pairs took 95.6 ms instead of 74.4 ms.
+constexpr int kLsWaves = 4;
+#define OF_ITER_BOUNDS(NW) __launch_bounds__(64 * (NW), (NW) == 4 ? kIterBlocksPerCU : 2)
 
 // ---------------------------------------------------------------------------
 // launch-per-chunk driver
@@ -300,13 +323,13 @@ constexpr int kIterBlocksPerCU = 3;
 // iteration that meets it falls inside the launch before its last
 // iteration, the launch's output is discarded and the next launch re-runs
 // exactly that many iterations from the same input buffer.
-template <int K, int R>
-__global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_iter(IterArgs a)
+template <int K, int R, int NW = 4>
+__global__ OF_ITER_BOUNDS(NW) void k_tvl1_iter(IterArgs a)
 {
-    __shared__ TileShared sh;
-    __shared__ double sh_err[4][K];
+    __shared__ TileSharedN<NW> sh;
+    __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
-    __shared__ double sh4[4];
+    __shared__ double sh4[NW];
     __shared__ int sh_last;
 
     TvCtl* ctl = a.ctl;
@@ -321,11 +344,11 @@ __global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_iter(IterArgs a)
     double acc[K];
 #pragma unroll
     for (int t = 0; t < K; t++) acc[t] = 0.0;
-    tile_iterate<K, R, false>(a, blockIdx.x, blockIdx.y, cur, kk, acc, sh);
+    tile_iterate<K, R, false, NW>(a, blockIdx.x, blockIdx.y, cur, kk, acc, sh);
 
     const unsigned nblocks = gridDim.x * gridDim.y;
     const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
-    const double part = block_sum<K>(acc, sh_err, threadIdx.x);
+    const double part = block_sum<K, NW>(acc, sh_err, threadIdx.x);
     // The partials are published with device-scope atomic stores, which are
     // written through to the point of coherence of all XCDs, and every wave
     // waits for their completion (s_waitcnt vmcnt(0)) before the counter
@@ -343,7 +366,7 @@ __global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_iter(IterArgs a)
     if (!sh_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int t = 0; t < kk; t++) {
-        const double tot = grid_sum(a.partials + (size_t)t * nblocks, nblocks, sh4);
+        const double tot = grid_sum<NW>(a.partials + (size_t)t * nblocks, nblocks, sh4);
         if (threadIdx.x == 0) sh_tot[t] = tot;
     }
     __syncthreads();
@@ -439,13 +462,13 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
 // with device-scope atomic accesses so that halos written by workgroups on
 // other XCDs are seen without cache maintenance.  Partials are
 // double-buffered by chunk parity.
-template <int K, int R>
-__global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_persist(IterArgs a, unsigned tiles_x, unsigned ntiles)
+template <int K, int R, int NW = 4>
+__global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, unsigned ntiles)
 {
-    __shared__ TileShared sh;
-    __shared__ double sh_err[4][K];
+    __shared__ TileSharedN<NW> sh;
+    __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
-    __shared__ double sh4[4];
+    __shared__ double sh4[NW];
     __shared__ int sh_ok;
 
     TvCtl* ctl = a.ctl;
@@ -464,20 +487,20 @@ __global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_persist(IterArgs
 #pragma unroll
         for (int t = 0; t < K; t++) acc[t] = 0.0;
         for (unsigned tile = blockIdx.x; tile < ntiles; tile += nb)
-            tile_iterate<K, R, true>(a, (int)(tile % tiles_x), (int)(tile / tiles_x), cur, kk, acc, sh);
+            tile_iterate<K, R, true, NW>(a, (int)(tile % tiles_x), (int)(tile / tiles_x), cur, kk, acc, sh);
         if (redo) {   // the re-run chunk is final: no decision needed
             n += redo;
             cur ^= 1;
             break;
         }
-        const double part = block_sum<K>(acc, sh_err, threadIdx.x);
+        const double part = block_sum<K, NW>(acc, sh_err, threadIdx.x);
         double* pp = a.partials + (size_t)parity * K * nb;
         if (threadIdx.x < (unsigned)kk)
             __hip_atomic_store(pp + (size_t)threadIdx.x * nb + blockIdx.x, part, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (!grid_barrier(ctl, nb, gen, &sh_ok)) return;
         for (int t = 0; t < kk; t++) {
-            const double tot = grid_sum(pp + (size_t)t * nb, nb, sh4);
+            const double tot = grid_sum<NW>(pp + (size_t)t * nb, nb, sh4);
             if (threadIdx.x == 0) sh_tot[t] = tot;
         }
         __syncthreads();
@@ -509,21 +532,24 @@ __global__ __launch_bounds__(256, kIterBlocksPerCU) void k_tvl1_persist(IterArgs
 // lockstep driver: one scale of many pairs per launch
 // ---------------------------------------------------------------------------
 
-// Inclusive prefix sums of v[0..n), n <= kLsMaxPairs, by one 256-thread
-// workgroup (Hillis-Steele over LDS).
+// Inclusive prefix sums of v[0..n), n <= kLsMaxPairs, by one workgroup of
+// NT threads (Hillis-Steele over LDS).
+template <int NT>
 __device__ __forceinline__ void ls_block_scan(unsigned* v, int n)
 {
+    static_assert(kLsMaxPairs % NT == 0, "pairs per thread");
+    constexpr int PER = kLsMaxPairs / NT;
     for (int off = 1; off < n; off <<= 1) {
-        unsigned t[kLsMaxPairs / 256];
+        unsigned t[PER];
 #pragma unroll
-        for (int k = 0; k < kLsMaxPairs / 256; k++) {
-            const int e = (int)threadIdx.x + 256 * k;
+        for (int k = 0; k < PER; k++) {
+            const int e = (int)threadIdx.x + NT * k;
             t[k] = (e < n && e >= off) ? v[e - off] : 0u;
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kLsMaxPairs / 256; k++) {
-            const int e = (int)threadIdx.x + 256 * k;
+        for (int k = 0; k < PER; k++) {
+            const int e = (int)threadIdx.x + NT * k;
             if (e < n) v[e] += t[k];
         }
         __syncthreads();
@@ -533,9 +559,9 @@ __device__ __forceinline__ void ls_block_scan(unsigned* v, int n)
 // Iterations of one pair's tile `bid` (kk <= K of them) and, in the
 // workgroup that finishes the pair's last tile of this launch, the pair's
 // stopping decision: the same partial sums, order and rule as k_tvl1_iter.
-template <int K, int R>
+template <int K, int R, int NW>
 __device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsigned bid,
-                                        TileShared& sh, double (*sh_err)[K], double* sh_tot,
+                                        TileSharedN<NW>& sh, double (*sh_err)[K], double* sh_tot,
                                         double* sh4, int* sh_last)
 {
     // the table entry is read in place: a private copy would be indexed by
@@ -551,8 +577,8 @@ __device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsi
     double acc[K];
 #pragma unroll
     for (int t = 0; t < K; t++) acc[t] = 0.0;
-    if (kk > 0) tile_iterate<K, R, false>(a, (int)(bid % P->gx), (int)(bid / P->gx), cur, kk, acc, sh);
-    const double part = block_sum<K>(acc, sh_err, threadIdx.x);
+    if (kk > 0) tile_iterate<K, R, false, NW>(a, (int)(bid % P->gx), (int)(bid / P->gx), cur, kk, acc, sh);
+    const double part = block_sum<K, NW>(acc, sh_err, threadIdx.x);
     if ((int)threadIdx.x < kk)
         __hip_atomic_store(a.partials + (size_t)threadIdx.x * nblocks + bid, part, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -564,7 +590,7 @@ __device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsi
     if (*sh_last) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         for (int t = 0; t < kk; t++) {
-            const double tot = grid_sum(a.partials + (size_t)t * nblocks, nblocks, sh4);
+            const double tot = grid_sum<NW>(a.partials + (size_t)t * nblocks, nblocks, sh4);
             if (threadIdx.x == 0) sh_tot[t] = tot;
         }
         __syncthreads();
@@ -600,23 +626,24 @@ __device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsi
     __syncthreads();
 }
 
-template <int K, int R>
-__global__ __launch_bounds__(256, kIterBlocksPerCU) void k_ls_iter(LsPhase ph, const LsPair* __restrict__ pairs,
-                                                 LsCtl* mc, unsigned long long* host_status,
-                                                 unsigned epoch)
+template <int K, int R, int NW>
+__global__ OF_ITER_BOUNDS(NW) void k_ls_iter(LsPhase ph, const LsPair* __restrict__ pairs,
+                                             LsCtl* mc, unsigned long long* host_status,
+                                             unsigned epoch)
 {
     // `pairs` is ph.pairs as a restrict kernel argument: the compiler may
     // then read the (wave-uniform) table entries with scalar loads
-    __shared__ TileShared sh;
-    __shared__ double sh_err[4][K];
+    constexpr int NT = 64 * NW;
+    __shared__ TileSharedN<NW> sh;
+    __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
-    __shared__ double sh4[4];
+    __shared__ double sh4[NW];
     __shared__ int sh_last;
     __shared__ unsigned s_pair[kLsMaxPairs], s_base[kLsMaxPairs];
 
     const unsigned nlist = mc->nlist;
     const unsigned ntot = mc->ntiles;
-    for (unsigned k = threadIdx.x; k < nlist; k += 256) {
+    for (unsigned k = threadIdx.x; k < nlist; k += NT) {
         s_pair[k] = mc->list_pair[k];
         s_base[k] = mc->list_base[k];
     }
@@ -632,7 +659,7 @@ __global__ __launch_bounds__(256, kIterBlocksPerCU) void k_ls_iter(LsPhase ph, c
         // fields) in scalar registers
         const unsigned pk = __builtin_amdgcn_readfirstlane(s_pair[lo]);
         const unsigned bid = __builtin_amdgcn_readfirstlane(t - s_base[lo]);
-        ls_tile<K, R>(ph, pairs + pk, bid, sh, sh_err, sh_tot, sh4, &sh_last);
+        ls_tile<K, R, NW>(ph, pairs + pk, bid, sh, sh_err, sh_tot, sh4, &sh_last);
     }
 
     // launch completion: the last workgroup rebuilds the work list
@@ -644,16 +671,16 @@ __global__ __launch_bounds__(256, kIterBlocksPerCU) void k_ls_iter(LsPhase ph, c
     if (!sh_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     const int np = ph.npairs;
-    for (int k = threadIdx.x; k < np; k += 256) {
+    for (int k = threadIdx.x; k < np; k += NT) {
         const int stop =
             __hip_atomic_load(&ph.pairs[k].a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_pair[k] = stop ? 0u : 1u;
         s_base[k] = stop ? 0u : ph.pairs[k].ntiles;
     }
     __syncthreads();
-    ls_block_scan(s_pair, np);
-    ls_block_scan(s_base, np);
-    for (int k = threadIdx.x; k < np; k += 256) {
+    ls_block_scan<NT>(s_pair, np);
+    ls_block_scan<NT>(s_base, np);
+    for (int k = threadIdx.x; k < np; k += NT) {
         const unsigned before = k ? s_pair[k - 1] : 0u;
         if (s_pair[k] != before) {   // pair k still iterates
             mc->list_pair[before] = (unsigned)k;
@@ -697,7 +724,8 @@ static hipError_t launch_kr(const IterArgs& a, hipStream_t s)
 {
     unsigned gx, gy;
     iter_grid(K, R, a.nx, a.ny, &gx, &gy);
-    hipLaunchKernelGGL((k_tvl1_iter<K, R>), dim3(gx, gy), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_tvl1_iter<K, R * 4 / kIterWaves, kIterWaves>), dim3(gx, gy),
+                       dim3(64 * kIterWaves), 0, s, a);
     return hipGetLastError();
 }
 
@@ -726,13 +754,15 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tvl1_persist<K, R>,
-                                                         256, 0) != hipSuccess) {
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>,
+                64 * kIterWaves, 0) != hipSuccess) {
             (void)hipGetLastError();
             return OF_ERR_HIP;
         }
         // at most 4 workgroups per CU: 1024 arrivals on one counter and
-        // pollers of one word per barrier are already plenty
+        // pollers of one word per barrier are already plenty (the compiled
+        // bound allows 2 of 8 waves)
         resident = coop ? std::min(per_cu, 4) * cus : 0;
     }
     if (resident <= 0) return OF_ERR_UNSUPPORTED;
@@ -745,8 +775,9 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
     IterArgs args = a;
     unsigned tx = gx, nt = ntiles;
     void* kargs[] = {(void*)&args, (void*)&tx, (void*)&nt};
-    const hipError_t e = hipLaunchCooperativeKernel((const void*)k_tvl1_persist<K, R>, dim3(nb),
-                                                    dim3(256), kargs, 0, s);
+    const hipError_t e =
+        hipLaunchCooperativeKernel((const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>,
+                                   dim3(nb), dim3(64 * kIterWaves), kargs, 0, s);
     if (e != hipSuccess) {
         record_hip_error(e, "k_tvl1_persist cooperative launch", __FILE__, __LINE__);
         return OF_ERR_HIP;
@@ -769,8 +800,8 @@ template <int K, int R>
 static hipError_t launch_ls_kr(const LsPhase& ph, LsCtl* mc, unsigned grid,
                                unsigned long long* host_status, unsigned epoch, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_ls_iter<K, R>), dim3(grid), dim3(256), 0, s, ph, ph.pairs, mc, host_status,
-                       epoch);
+    hipLaunchKernelGGL((k_ls_iter<K, R * 4 / kLsWaves, kLsWaves>), dim3(grid),
+                       dim3(64 * kLsWaves), 0, s, ph, ph.pairs, mc, host_status, epoch);
     return hipGetLastError();
 }
 
@@ -798,8 +829,9 @@ static int ls_resident_kr(unsigned* blocks)
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ls_iter<K, R>, 256, 0) !=
-            hipSuccess) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void*)k_ls_iter<K, R * 4 / kLsWaves, kLsWaves>, 64 * kLsWaves,
+            0) != hipSuccess) {
         (void)hipGetLastError();
         return OF_ERR_HIP;
     }
