@@ -83,7 +83,8 @@ BATCH_SIZES = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
 def batched(npairs: int, lanes: int, reps: int, workers: bool = True):
     """64 pairs whose sizes are drawn uniformly (seeded) from BATCH_SIZES,
     same generator and parameters as the 1920x1080 workload, solved by the
-    lockstep batch solver (and, for comparison, its worker mode); two
+    lockstep batch solver with launch shapes from its autotuner (and, for
+    comparison, its worker mode); two
     generated pairs per size are cycled through.  The algorithmic bytes of
     the batch are the sum of its pairs' (each pair solved once alone for its
     iteration counts, which the batch reproduces exactly)."""
@@ -117,8 +118,17 @@ def batched(npairs: int, lanes: int, reps: int, workers: bool = True):
         tv.close()
     up = [t.data_ptr() for t in dU]
 
+    tune = {}
+
     def timed(mode, streams):
         with TVL1Batch(streams=streams, nscales=5, mode=mode) as b:
+            if mode == "lockstep":
+                # launch shapes from the batch autotuner (untimed)
+                shapes, table, best = b.autotune(ws, hs, d0, d1, up, reps=2)
+                tune.update(shapes={"large_scales": list(shapes[0]), "small_scales": list(shapes[1]),
+                                    "split_pixels": 200000},
+                            uniform_ms={f"{k}x{r}": round(v, 2) for (k, r), v in table.items()},
+                            chosen_ms=round(best, 2))
             b.run_device_u8(ws, hs, d0, d1, up)                 # warm-up (handles, tables)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -143,6 +153,7 @@ def batched(npairs: int, lanes: int, reps: int, workers: bool = True):
                 "streams": 4, "ms_per_batch": round(dt_w * 1e3, 2),
                 "hbm_roofline_frac": round(B / dt_w / 8e12, 4)},
             "iterations_match_single_solves": same,
+            "launch_shapes": tune,
             "data": "synthetic, 2 generated pairs per size cycled through the batch"}
 
 

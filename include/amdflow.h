@@ -193,6 +193,20 @@ int of_tvl1_batch_set_mode(of_tvl1_batch *b, int mode);
  * at each scale, in worker mode applied to every handle. */
 int of_tvl1_batch_set_shape(of_tvl1_batch *b, long long min_pixels, int iters_per_launch,
                             int rows_per_wave);
+/* Launch-shape autotuner of the batch (no reference counterpart).  Solves
+ * the given batch with each of OF_BATCH_TUNE_SHAPES shapes (iterations per
+ * launch, rows per wave) = (4,8), (2,8), (8,8), (4,4), (2,4) at every scale,
+ * then with the three fastest paired (one at scales of at least 200000
+ * pixels, another below); each variant once untimed and `reps` times timed.
+ * Installs the fastest (as of_tvl1_batch_set_shape would) and returns it in
+ * chosen[4] = {K large, R large, K small, R small}; ms_table (optional,
+ * OF_BATCH_TUNE_SHAPES entries) receives the uniform shapes' ms per batch,
+ * best_ms (optional) the chosen variant's.  The flows in dU are those of a
+ * solve (every shape gives the same bits). */
+#define OF_BATCH_TUNE_SHAPES 5
+int of_tvl1_batch_autotune(of_tvl1_batch *b, int npairs, const int *widths, const int *heights,
+                           const uint8_t *const *dI0, const uint8_t *const *dI1, float *const *dU,
+                           int reps, int *chosen, double *ms_table, double *best_ms);
 /* Solves pair i: device u8 frames dI0[i], dI1[i] of widths[i] x heights[i],
  * flow (planar [u1|u2]) into device buffer dU[i].  `iterations_total`
  * (optional) receives each pair's total primal-dual iteration count.

@@ -23,6 +23,12 @@ _lib.of_tvl1_batch_run_u8.argtypes = [_vp, C.c_int, _ip, _ip, C.POINTER(_vp), C.
                                       C.POINTER(_vp), _ip]
 
 
+_lib.of_tvl1_batch_autotune.restype = C.c_int
+_lib.of_tvl1_batch_autotune.argtypes = [_vp, C.c_int, _ip, _ip, C.POINTER(_vp), C.POINTER(_vp),
+                                        C.POINTER(_vp), C.c_int, _ip, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
+TUNE_SHAPES = [(4, 8), (2, 8), (8, 8), (4, 4), (2, 4)]
+
 _lib.of_tvl1_run_batch.restype = C.c_int
 _lib.of_tvl1_run_batch.argtypes = [C.POINTER(TVL1Params), C.c_int, _ip, _ip, C.POINTER(_vp),
                                    C.POINTER(_vp), C.POINTER(_vp), _ip]
@@ -65,6 +71,25 @@ class TVL1Batch:
     def set_shape(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int) -> None:
         _check(_lib.of_tvl1_batch_set_shape(self._h, int(min_pixels), int(iters_per_launch),
                                             int(rows_per_wave)), "batch_set_shape")
+
+    def autotune(self, widths, heights, d_I0, d_I1, d_U, reps: int = 2):
+        """of_tvl1_batch_autotune on this batch: installs and returns the
+        fastest shapes ((K, R) at scales of >= 200000 pixels, (K, R) below),
+        the uniform shapes' ms per batch (dict over TUNE_SHAPES) and the
+        chosen variant's ms."""
+        n = len(widths)
+        w = np.ascontiguousarray(widths, np.int32)
+        h = np.ascontiguousarray(heights, np.int32)
+        chosen = np.zeros(4, np.int32)
+        table = np.zeros(len(TUNE_SHAPES), np.float64)
+        best = C.c_double()
+        _check(_lib.of_tvl1_batch_autotune(self._h, n, w.ctypes.data_as(_ip), h.ctypes.data_as(_ip),
+                                           (_vp * n)(*d_I0), (_vp * n)(*d_I1), (_vp * n)(*d_U),
+                                           int(reps), chosen.ctypes.data_as(_ip),
+                                           table.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.byref(best)), "batch_autotune")
+        c = [int(v) for v in chosen]
+        return ((c[0], c[1]), (c[2], c[3])), dict(zip(TUNE_SHAPES, table.tolist())), best.value
 
     def run_device_u8(self, widths, heights, d_I0, d_I1, d_U):
         """d_I0/d_I1/d_U: sequences of device pointers (ints).  Returns the

@@ -27,6 +27,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -85,6 +86,11 @@ struct of_tvl1_batch {
     std::map<std::pair<int, int>, std::vector<of_tvl1*>> pool;   // handles by frame size
     std::vector<LsLane> lanes;
 };
+
+// Batch autotuner: the candidate shapes, and the scale size (pixels) that
+// separates "large" from "small" scales when two shapes are combined.
+constexpr int kBatchTuneShapes = OF_BATCH_TUNE_SHAPES;
+constexpr long long kBatchTuneSplit = 200000;
 
 namespace {
 
@@ -710,6 +716,81 @@ int of_tvl1_batch_set_shape(of_tvl1_batch* b, long long min_pixels, int iters_pe
     for (auto& m : b->handles)
         for (auto& kv : m) of_tvl1_destroy(kv.second);
     for (auto& m : b->handles) m.clear();
+    return OF_OK;
+}
+
+int of_tvl1_batch_autotune(of_tvl1_batch* b, int npairs, const int* widths, const int* heights,
+                           const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
+                           int reps, int* chosen, double* ms_table, double* best_ms)
+{
+    if (!b || !chosen || reps < 1 || npairs < 1) return OF_ERR_ARG;
+    // candidate shapes (iterations per launch, rows per wave), the default first
+    static const int kCand[kBatchTuneShapes][2] = {{4, 8}, {2, 8}, {8, 8}, {4, 4}, {2, 4}};
+    using Rules = std::vector<std::tuple<long long, int, int>>;
+    const Rules saved = b->rules;
+    auto install = [&](const Rules& r) {
+        b->rules = r;
+        for (auto& m : b->handles)
+            for (auto& kv : m) of_tvl1_destroy(kv.second);
+        for (auto& m : b->handles) m.clear();
+    };
+    // one untimed run (first launches of a shape load its code), then `reps`
+    // timed ones; the runs are synchronous, so the host clock brackets them
+    auto time_rules = [&](const Rules& r, double* ms) -> int {
+        try {
+            install(r);
+        } catch (...) {
+            return OF_ERR_ALLOC;
+        }
+        OF_TRY(of_tvl1_batch_run_u8(b, npairs, widths, heights, dI0, dI1, dU, nullptr));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < reps; k++)
+            OF_TRY(of_tvl1_batch_run_u8(b, npairs, widths, heights, dI0, dI1, dU, nullptr));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+              reps;
+        return OF_OK;
+    };
+    double t[kBatchTuneShapes];
+    int st = OF_OK;
+    for (int c = 0; c < kBatchTuneShapes && !st; c++)
+        st = time_rules(Rules{{0, kCand[c][0], kCand[c][1]}}, &t[c]);
+    if (st) {
+        install(saved);
+        return st;
+    }
+    int order[kBatchTuneShapes];
+    for (int c = 0; c < kBatchTuneShapes; c++) order[c] = c;
+    std::sort(order, order + kBatchTuneShapes, [&](int x, int y) { return t[x] < t[y]; });
+    // then the three fastest uniform shapes paired: one at scales of at least
+    // kBatchTuneSplit pixels, another below
+    int big = order[0], small = order[0];
+    double best = t[order[0]];
+    for (int i = 0; i < 3 && !st; i++)
+        for (int j = 0; j < 3 && !st; j++) {
+            if (i == j) continue;
+            const int x = order[i], y = order[j];
+            double ms = 0;
+            st = time_rules(Rules{{kBatchTuneSplit, kCand[x][0], kCand[x][1]},
+                                  {0, kCand[y][0], kCand[y][1]}},
+                            &ms);
+            if (!st && ms < best) {
+                best = ms;
+                big = x;
+                small = y;
+            }
+        }
+    if (st) {
+        install(saved);
+        return st;
+    }
+    install(Rules{{kBatchTuneSplit, kCand[big][0], kCand[big][1]}, {0, kCand[small][0], kCand[small][1]}});
+    chosen[0] = kCand[big][0];
+    chosen[1] = kCand[big][1];
+    chosen[2] = kCand[small][0];
+    chosen[3] = kCand[small][1];
+    if (ms_table)
+        for (int c = 0; c < kBatchTuneShapes; c++) ms_table[c] = t[c];
+    if (best_ms) *best_ms = best;
     return OF_OK;
 }
 

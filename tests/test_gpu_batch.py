@@ -124,3 +124,25 @@ def test_run_batch_one_call_matches_batch_object():
     assert np.array_equal(it, it2)
     for k in range(len(sizes)):
         assert np.array_equal(dU[k].cpu().numpy(), flows[k]), k
+
+
+def test_batch_autotune_installs_a_candidate_and_stays_exact():
+    """of_tvl1_batch_autotune times the candidate shapes on the batch,
+    installs the fastest, and later runs still match the oracle."""
+    from amdflow.batch import TUNE_SHAPES, TVL1Batch
+    sizes = [(320, 240), (160, 120), (640, 480), (97, 61), (256, 192)]
+    frames, d0, d1 = _frames(sizes, 3.0, 700)
+    with TVL1Batch(streams=3) as b:
+        import torch
+        dU = [torch.empty(2 * w * h, dtype=torch.float32, device="cuda") for (w, h) in sizes]
+        (big, small), table, best = b.autotune([w for w, _ in sizes], [h for _, h in sizes],
+                                               [t.data_ptr() for t in d0], [t.data_ptr() for t in d1],
+                                               [t.data_ptr() for t in dU], reps=1)
+        assert big in TUNE_SHAPES and small in TUNE_SHAPES
+        assert set(table) == set(TUNE_SHAPES) and all(v > 0 for v in table.values())
+        assert 0 < best <= min(table.values())
+        it, flows = _run(b, sizes, d0, d1)
+    for k, (q0, q1) in enumerate(frames):
+        uo, ito = O.tvl1(q0, q1)
+        assert it[k] == int(ito.sum()), k
+        assert np.array_equal(flows[k], uo), k
