@@ -174,19 +174,25 @@ __global__ __launch_bounds__(256) void k_lk_tensor_products(const float* __restr
     t3[2 * n + p] = gy * gy;
 }
 
+// Output rows of the fused LK tiles (64 columns wide).  8 rows give twice
+// the workgroups of 16 for the same work, which the coarse levels need: the
+// 1280x720 3-level solve took 0.260 ms with 8 everywhere, 0.265 ms with 8
+// below 300,000 pixels and 16 above, 0.290 ms with 16 everywhere.
+constexpr int kLkTileRows = 8;
+
 // The gradient, its products and their window sums in one launch, on a
-// 64 x 16 output tile: the products on the tile plus a WC-pixel halo (LDS),
+// 64 x kLkTileRows output tile: the products on the tile plus a WC-pixel halo (LDS),
 // their row sums, then the column sums into G (Gxx, Gxy, Gyy planes); Ix and
 // Iy are written for the output pixels.  Same expressions and summation
 // order as k_lk_tensor_products followed by k_lk_boxsum along rows and then
 // columns (windows clipped to the image, ascending order).
-template <int WC>
+template <int WC, int TH = kLkTileRows>
 __global__ __launch_bounds__(256) void k_lk_tensor_fused(const float* __restrict__ P0, int w,
                                                          int h, float* __restrict__ Ix,
                                                          float* __restrict__ Iy,
                                                          float* __restrict__ G)
 {
-    constexpr int TW = 64, TH = 16, EW = TW + 2 * WC, EH = TH + 2 * WC;
+    constexpr int TW = 64, EW = TW + 2 * WC, EH = TH + 2 * WC;
     constexpr int NE = (EW * EH + 255) / 256;
     constexpr int NO = TW * TH / 256;
     __shared__ float pr[3][EH][EW];
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ row
     }
 }
 
-// One whole iteration on a 64 x 16 output tile: the warp and the two
+// One whole iteration on a 64 x kLkTileRows output tile: the warp and the two
 // products on the tile plus a WC-pixel halo (LDS), their row sums, then the
 // column sums and the 2x2 update.  Same expressions and summation order as
 // k_lk_it_products / k_lk_boxsum / k_lk_update (products outside the image
@@ -344,7 +350,7 @@ __global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ row
 // issued before the first use: the output pixels' flow and tensor, then the
 // extended tile's flow, then its samples, so one launch waits on two memory
 // round trips, not one per element.
-template <int WC>
+template <int WC, int TH = kLkTileRows>
 __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__ P0,
                                                        const float* __restrict__ P1,
                                                        const float* __restrict__ Ix,
@@ -355,7 +361,7 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
                                                        float det_eps, float* __restrict__ uo,
                                                        float* __restrict__ vo)
 {
-    constexpr int TW = 64, TH = 16, EW = TW + 2 * WC, EH = TH + 2 * WC;
+    constexpr int TW = 64, EW = TW + 2 * WC, EH = TH + 2 * WC;
     constexpr int NE = (EW * EH + 255) / 256;   // extended-tile elements per thread
     constexpr int NO = TW * TH / 256;           // output pixels per thread
     __shared__ float pr[2][EH][EW];
@@ -626,7 +632,7 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
             const int w = nx[l], h = ny[l];
             const float* p0 = L0[l];
             const float* p1 = L1[l];
-            const dim3 fgrid(cdiv(w, 64), cdiv(h, 16));
+            const dim3 fgrid(cdiv(w, 64), cdiv(h, kLkTileRows));
             if (wc >= 1 && wc <= 4) {
                 // gradient, products and both window sums in one launch
                 if (wc == 1) hipLaunchKernelGGL((k_lk_tensor_fused<1>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
