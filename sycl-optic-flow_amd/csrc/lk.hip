@@ -14,6 +14,7 @@
 // u -= G^-1 b where det G > det_eps), flow carried up with zoom_in x 1/zf.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -616,8 +617,11 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
         const int wc = (win - 1) / 2;
         // the last fused iteration at level 0 writes the caller's du/dv
         // directly unless they overlap the frames or each other
+        // address ranges of len floats, compared as integers (the buffers
+        // are separate allocations)
         auto overlaps = [](const float* a, const float* b, size_t len) {
-            return a < b + len && b < a + len;
+            const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b, n = len * sizeof(float);
+            return x < y + n && y < x + n;
         };
         const bool direct = wc >= 1 && wc <= 4 && iters >= 1 && !overlaps(du, dv, n0) &&
                             !overlaps(du, dI0, n0) && !overlaps(du, dI1, n0) &&
