@@ -262,25 +262,40 @@ __device__ __forceinline__ double block_sum(const double (&acc)[K], double (*sh_
     return s;
 }
 
-// Sum over all workgroups of column t of `part` (nb entries), fixed order:
-// thread j sums entries j, j+64*NW, ...; then wave trees and waves 0..NW-1
-// left to right.  Identical in every workgroup that calls it on the same
-// data.
-template <int NW = 4>
-__device__ __forceinline__ double grid_sum(const double* part, unsigned nb, double* sh4)
+// Sums over all workgroups of columns t < kk of `part` (column t holds nb
+// entries at part + t*nb), fixed order per column: thread j sums entries j,
+// j+64*NW, ...; then wave trees and waves 0..NW-1 left to right.  All
+// columns are read in one pass, so the sums cost one memory round trip.
+// Identical in every workgroup that calls it on the same data; out[t] is
+// valid in every thread after the call (sh: K*NW doubles of LDS).
+template <int K, int NW>
+__device__ __forceinline__ void grid_sums(const double* part, unsigned nb, int kk, double* sh,
+                                          double* out)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double s = 0.0;
-    for (unsigned j = threadIdx.x; j < nb; j += blockDim.x)
-        s += __hip_atomic_load(part + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = wave_sum_f64(s);
-    if (lane == 0) sh4[wave] = s;
-    __syncthreads();
-    double tot = sh4[0];
+    double s[K];
 #pragma unroll
-    for (int w = 1; w < NW; w++) tot += sh4[w];
+    for (int t = 0; t < K; t++) s[t] = 0.0;
+    for (unsigned j = threadIdx.x; j < nb; j += 64 * NW) {
+#pragma unroll
+        for (int t = 0; t < K; t++)
+            if (t < kk)
+                s[t] += __hip_atomic_load(part + (size_t)t * nb + j, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const double v = wave_sum_f64(s[t]);
+        if (lane == 0) sh[t * NW + wave] = v;
+    }
     __syncthreads();
-    return tot;
+    if (threadIdx.x < (unsigned)K) {
+        double tot = sh[threadIdx.x * NW];
+#pragma unroll
+        for (int w = 1; w < NW; w++) tot += sh[threadIdx.x * NW + w];
+        out[threadIdx.x] = tot;
+    }
+    __syncthreads();
 }
 
 // The stopping rule applied to kk error sums after iteration n0 (1-based
@@ -329,7 +344,7 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_iter(IterArgs a)
     __shared__ TileSharedN<NW> sh;
     __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
-    __shared__ double sh4[NW];
+    __shared__ double sh4[K * NW];
     __shared__ int sh_last;
 
     TvCtl* ctl = a.ctl;
@@ -365,11 +380,7 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_iter(IterArgs a)
     __syncthreads();
     if (!sh_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int t = 0; t < kk; t++) {
-        const double tot = grid_sum<NW>(a.partials + (size_t)t * nblocks, nblocks, sh4);
-        if (threadIdx.x == 0) sh_tot[t] = tot;
-    }
-    __syncthreads();
+    grid_sums<K, NW>(a.partials, nblocks, kk, sh4, sh_tot);
     if (a.err_log && threadIdx.x < (unsigned)kk) a.err_log[n0 + threadIdx.x] = sh_tot[threadIdx.x];
     if (threadIdx.x == 0) {
         int n = n0, stop = 0, nredo = 0, flip = 0;
@@ -468,7 +479,7 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
     __shared__ TileSharedN<NW> sh;
     __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
-    __shared__ double sh4[NW];
+    __shared__ double sh4[K * NW];
     __shared__ int sh_ok;
 
     TvCtl* ctl = a.ctl;
@@ -499,11 +510,7 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
             __hip_atomic_store(pp + (size_t)threadIdx.x * nb + blockIdx.x, part, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (!grid_barrier(ctl, nb, gen, &sh_ok)) return;
-        for (int t = 0; t < kk; t++) {
-            const double tot = grid_sum<NW>(pp + (size_t)t * nb, nb, sh4);
-            if (threadIdx.x == 0) sh_tot[t] = tot;
-        }
-        __syncthreads();
+        grid_sums<K, NW>(pp, nb, kk, sh4, sh_tot);
         const int keep = stop_after(sh_tot, kk, n, a);
         if (a.err_log && blockIdx.x == 0 && threadIdx.x < (unsigned)kk)
             a.err_log[n + threadIdx.x] = sh_tot[threadIdx.x];
@@ -589,11 +596,7 @@ __device__ __forceinline__ void ls_tile(const LsPhase& ph, const LsPair* P, unsi
     __syncthreads();
     if (*sh_last) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (int t = 0; t < kk; t++) {
-            const double tot = grid_sum<NW>(a.partials + (size_t)t * nblocks, nblocks, sh4);
-            if (threadIdx.x == 0) sh_tot[t] = tot;
-        }
-        __syncthreads();
+        grid_sums<K, NW>(a.partials, nblocks, kk, sh4, sh_tot);
         if (threadIdx.x == 0) {
             int n = n0, stop = 0, nredo = 0, flip = 0;
             if (kk <= 0) {   // nothing left to do: only reached if the list was stale
@@ -637,7 +640,7 @@ __global__ OF_ITER_BOUNDS(NW) void k_ls_iter(LsPhase ph, const LsPair* __restric
     __shared__ TileSharedN<NW> sh;
     __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
-    __shared__ double sh4[NW];
+    __shared__ double sh4[K * NW];
     __shared__ int sh_last;
     __shared__ unsigned s_pair[kLsMaxPairs], s_base[kLsMaxPairs];
 
