@@ -200,6 +200,21 @@ __global__ __launch_bounds__(256) void k_lk_tensor_fused(const float* __restrict
     __shared__ float rs[3][EH][TW];
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const size_t n = (size_t)w * h;
+    // the four neighbour loads of every element are issued without branches
+    // (outside the image: clamped, in-range addresses, result discarded)
+    float nl[NE], nr[NE], nu[NE], nd[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = threadIdx.x + 256 * k;
+        const int ey = e / EW, ex = e - ey * EW;
+        const int j = min(max(x0 - WC + ex, 0), w - 1), i = min(max(y0 - WC + ey, 0), h - 1);
+        const int jl = j > 0 ? j - 1 : 0, jr = j < w - 1 ? j + 1 : w - 1;
+        const int iu = i > 0 ? i - 1 : 0, id = i < h - 1 ? i + 1 : h - 1;
+        nl[k] = P0[(size_t)i * w + jl];
+        nr[k] = P0[(size_t)i * w + jr];
+        nu[k] = P0[(size_t)iu * w + j];
+        nd[k] = P0[(size_t)id * w + j];
+    }
 #pragma unroll
     for (int k = 0; k < NE; k++) {
         const int e = threadIdx.x + 256 * k;
@@ -208,10 +223,8 @@ __global__ __launch_bounds__(256) void k_lk_tensor_fused(const float* __restrict
         const int j = x0 - WC + ex, i = y0 - WC + ey;
         float a = 0.0f, b = 0.0f, c = 0.0f;
         if (j >= 0 && j < w && i >= 0 && i < h) {
-            const int jl = j > 0 ? j - 1 : 0, jr = j < w - 1 ? j + 1 : w - 1;
-            const int iu = i > 0 ? i - 1 : 0, id = i < h - 1 ? i + 1 : h - 1;
-            const float ddx = P0[(size_t)i * w + jr] - P0[(size_t)i * w + jl];
-            const float ddy = P0[(size_t)id * w + j] - P0[(size_t)iu * w + j];
+            const float ddx = nr[k] - nl[k];
+            const float ddy = nd[k] - nu[k];
             const float gx = (float)(0.5 * (double)ddx), gy = (float)(0.5 * (double)ddy);
             a = gx * gx;
             b = gx * gy;
@@ -400,22 +413,51 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
             ev[k] = v[p];
         }
     }
+    // every element's loads first, without branches (outside the image: a
+    // valid address, result discarded), so that they are all in flight
+    // together; then the arithmetic, as bilinear_at evaluates it
+    float ta[NE], tb[NE], tc[NE], td[NE], fx[NE], fy[NE], i0[NE], gxv[NE], gyv[NE];
+    bool in[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = threadIdx.x + 256 * k;
+        const int ey = e / EW, ex = e - ey * EW;
+        const int gx = x0 - WC + ex, gy = y0 - WC + ey;
+        in[k] = e < EW * EH && gx >= 0 && gx < w && gy >= 0 && gy < h;
+        const size_t p = in[k] ? (size_t)gy * w + gx : 0;
+        const float x = (float)gx + eu[k], y = (float)gy + ev[k];
+        const float fx0 = floorf(x), fy0 = floorf(y);
+        fx[k] = x - fx0;
+        fy[k] = y - fy0;
+        const int xi = (int)fx0, yi = (int)fy0;
+        const int xa = clampi(xi, 0, w - 1), xb = clampi(xi + 1, 0, w - 1);
+        const int ya = clampi(yi, 0, h - 1), yb = clampi(yi + 1, 0, h - 1);
+        ta[k] = P1[(size_t)ya * w + xa];
+        tb[k] = P1[(size_t)ya * w + xb];
+        tc[k] = P1[(size_t)yb * w + xa];
+        td[k] = P1[(size_t)yb * w + xb];
+        i0[k] = P0[p];
+        gxv[k] = Ix[p];
+        gyv[k] = Iy[p];
+    }
+    float ea[NE], eb[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const float ax = fx[k], ay = fy[k];
+        const float top = (1.0f - ax) * ta[k] + ax * tb[k];
+        const float bot = (1.0f - ax) * tc[k] + ax * td[k];
+        const float wv = (1.0f - ay) * top + ay * bot;
+        const float itv = wv - i0[k];
+        ea[k] = in[k] ? gxv[k] * itv : 0.0f;
+        eb[k] = in[k] ? gyv[k] * itv : 0.0f;
+    }
 #pragma unroll
     for (int k = 0; k < NE; k++) {
         const int e = threadIdx.x + 256 * k;
         if (e >= EW * EH) break;
         const int ey = e / EW, ex = e - ey * EW;
-        const int gx = x0 - WC + ex, gy = y0 - WC + ey;
-        float a = 0.0f, b = 0.0f;
-        if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-            const size_t p = (size_t)gy * w + gx;
-            const float wv = bilinear_at(P1, w, h, (float)gx + eu[k], (float)gy + ev[k]);
-            const float itv = wv - P0[p];
-            a = Ix[p] * itv;
-            b = Iy[p] * itv;
-        }
-        pr[0][ey][ex] = a;
-        pr[1][ey][ex] = b;
+        pr[0][ey][ex] = ea[k];
+        pr[1][ey][ex] = eb[k];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < TW * EH; e += 256) {
