@@ -188,7 +188,8 @@ def other_paths(cpu_budget: float):
     and pyramidal Lucas-Kanade 1280x720 (3 levels, zoom 0.5, sigma 1, 7x7,
     5 iterations, det > 1e-6) on float frames in HBM, with their oracle
     timed on the host; algorithmic bytes: HS 28 B per pixel-sweep + 52 B per
-    pixel-warping, LK 44 B per pixel-iteration + 24 B per pixel-level."""
+    pixel-warping, LK 44 B per pixel-iteration + 24 B per pixel-level.  Then
+    the single-scale LK through the reference's frame-by-frame interface."""
     import numpy as np
     import torch
 
@@ -236,6 +237,41 @@ def other_paths(cpu_budget: float):
     rec["bitwise_equal_to_oracle"] = bool(np.array_equal(du.cpu().numpy(), u_o.reshape(-1))
                                           and np.array_equal(dv.cpu().numpy(), v_o.reshape(-1)))
     out["lk"] = rec
+
+    # the reference's own LK interface (lucaskanade.cuh:8-11) as its application
+    # drives it (src/lk/cuda/main.cu:132-160: spatial filter 3, temporal 2,
+    # window 5): each step pushes one host u8 frame into the ring and returns
+    # the flow to the host, so the time includes both PCIe transfers
+    from amdflow import LucasKanade
+    q = [synth.quantize(I0), synth.quantize(I1)]
+    Vx = np.zeros(W * H, np.float32)
+    Vy = np.zeros(W * H, np.float32)
+    lk = LucasKanade(3, 2, 5, W, H)
+    lk.copy_frames_circularbuffer_GPU_wrapper(q[0], 2, 0, W * H)
+    frame = [1]
+
+    def lk_step():
+        fr = frame[0]
+        lk.copy_frames_circularbuffer_GPU_wrapper(q[fr % 2], 2, fr, W * H)
+        lk.lucas_kanade(Vx, Vy, fr)
+        frame[0] = fr + 1
+
+    t = _timed(lk_step, 50)
+    rec = {"workload": "lk_stream_1280x720", "interface": "LucasKanade push + lucas_kanade (host frames)",
+           "frames_per_s": round(1 / t, 1), "ms_per_frame": round(t * 1e3, 4),
+           "pcie_bytes_per_frame": W * H * (1 + 8)}
+    ring = np.zeros((2, H, W), np.uint8)
+    ring[0], ring[1] = q[0], q[1]
+    tc, runs = _cpu_timed(lambda: O.lk_single(ring, 1, 3, 5), cpu_budget)
+    rec["cpu_baseline"] = {"value": round(1 / tc, 2), "unit": "frames/s", "kind": "port",
+                           "cores": _cores(), "sample": f"{runs} frame(s), OpenMP"}
+    lk.copy_frames_circularbuffer_GPU_wrapper(q[0], 2, 0, W * H)
+    lk.copy_frames_circularbuffer_GPU_wrapper(q[1], 2, 1, W * H)
+    lk.lucas_kanade(Vx, Vy, 1)
+    ox, oy = O.lk_single(ring, 1, 3, 5)
+    rec["bitwise_equal_to_oracle"] = bool(np.array_equal(Vx, ox.reshape(-1)) and np.array_equal(Vy, oy.reshape(-1)))
+    lk.close()
+    out["lk_stream"] = rec
     return out
 
 
