@@ -719,11 +719,10 @@ int of_tvl1_batch_set_shape(of_tvl1_batch* b, long long min_pixels, int iters_pe
     return OF_OK;
 }
 
-int of_tvl1_batch_autotune(of_tvl1_batch* b, int npairs, const int* widths, const int* heights,
-                           const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
-                           int reps, int* chosen, double* ms_table, double* best_ms)
+static int batch_autotune(of_tvl1_batch* b, int npairs, const int* widths, const int* heights,
+                          const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
+                          int reps, int* chosen, double* ms_table, double* best_ms)
 {
-    if (!b || !chosen || reps < 1 || npairs < 1) return OF_ERR_ARG;
     // candidate shapes (iterations per launch, rows per wave), the default first
     static const int kCand[kBatchTuneShapes][2] = {{4, 8}, {2, 8}, {8, 8}, {4, 4}, {2, 4}};
     using Rules = std::vector<std::tuple<long long, int, int>>;
@@ -792,6 +791,20 @@ int of_tvl1_batch_autotune(of_tvl1_batch* b, int npairs, const int* widths, cons
         for (int c = 0; c < kBatchTuneShapes; c++) ms_table[c] = t[c];
     if (best_ms) *best_ms = best;
     return OF_OK;
+}
+
+int of_tvl1_batch_autotune(of_tvl1_batch* b, int npairs, const int* widths, const int* heights,
+                           const uint8_t* const* dI0, const uint8_t* const* dI1, float* const* dU,
+                           int reps, int* chosen, double* ms_table, double* best_ms)
+{
+    if (!b || !chosen || reps < 1 || npairs < 1) return OF_ERR_ARG;
+    // the C ABI never throws: allocation failures become OF_ERR_ALLOC
+    try {
+        return batch_autotune(b, npairs, widths, heights, dI0, dI1, dU, reps, chosen, ms_table,
+                              best_ms);
+    } catch (...) {
+        return OF_ERR_ALLOC;
+    }
 }
 
 int of_tvl1_batch_set_mode(of_tvl1_batch* b, int mode)

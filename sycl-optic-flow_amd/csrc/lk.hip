@@ -560,12 +560,12 @@ unsigned g1(size_t n) { const unsigned g = cdiv(n, 256); return g > 4096 ? 4096 
 // handle).
 struct LkWorkspace {
     size_t cap_total = 0, cap_n0 = 0;
-    float *P0 = nullptr, *P1 = nullptr, *A0 = nullptr, *A1 = nullptr, *Ix = nullptr, *Iy = nullptr;
+    float *P0 = nullptr, *P1 = nullptr, *Ix = nullptr, *Iy = nullptr;
     float *PROD = nullptr, *ROWS = nullptr, *GT = nullptr, *u = nullptr, *v = nullptr, *nu = nullptr,
           *nv = nullptr;
     void release()
     {
-        for (float** p : {&P0, &P1, &A0, &A1, &Ix, &Iy, &PROD, &ROWS, &GT, &u, &v, &nu, &nv}) dev_free(*p);
+        for (float** p : {&P0, &P1, &Ix, &Iy, &PROD, &ROWS, &GT, &u, &v, &nu, &nv}) dev_free(*p);
         cap_total = cap_n0 = 0;
     }
     int ensure(size_t total, size_t n0)
@@ -575,7 +575,7 @@ struct LkWorkspace {
         release();
         int st = dev_alloc(&P0, total);
         if (!st) st = dev_alloc(&P1, total);
-        for (float** p : {&A0, &A1, &Ix, &Iy, &u, &v, &nu, &nv})
+        for (float** p : {&Ix, &Iy, &u, &v, &nu, &nv})
             if (!st) st = dev_alloc(p, n0);
         for (float** p : {&PROD, &ROWS, &GT})
             if (!st) st = dev_alloc(p, 3 * n0);
@@ -622,8 +622,6 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
     int status = ws.ensure(total, n0);
     float* P0 = ws.P0;
     float* P1 = ws.P1;
-    float* A0 = ws.A0;
-    float* A1 = ws.A1;
     float* Ix = ws.Ix;
     float* Iy = ws.Iy;
     float* PROD = ws.PROD;   // per-pixel products (3 or 2 planes of the level size)
