@@ -326,7 +326,7 @@ constexpr int kIterBlocksPerCU = 3;
 // 9.29 -> 8.18 us).  Compiled for 2 workgroups per CU (128 VGPRs).
 constexpr int kIterWaves = 8;
 // The lockstep batch kernel keeps 4 waves of R rows: with 8 waves of R/2 the
-// batch of 64 mixed-size pairs took 95.6 ms instead of 74.4 ms.
+// batch of 64 mixed-size pairs took 79.0 ms instead of 75.7 ms (same box).
 constexpr int kLsWaves = 4;
 #define OF_ITER_BOUNDS(NW) __launch_bounds__(64 * (NW), (NW) == 4 ? kIterBlocksPerCU : 2)
 
