@@ -311,6 +311,13 @@ def warm_start(npairs: int):
     net = cnn.WarmStartNet.pretrained()
     res = evaluate(net, npairs=npairs)
     res["workload"] = f"tvl1_warm_1920x1080_batch{npairs}"
+    # the setting whose mean EPE against the plain flow is within 0.05 px
+    # (the metric's bar): seed 240x135 and keep all 5 warpings
+    tight = evaluate(net, npairs=npairs, first_scale=3, warps=5)
+    res["at_epe_0_05"] = {k: tight[k] for k in ("first_scale", "first_scale_size", "warps",
+                                                "max_iterations", "pairs_per_s",
+                                                "speedup_warm_vs_plain", "mean_epe_vs_plain",
+                                                "fallbacks") if k in tight}
     res["network"] = {"layers": f"conv3x3 5-32, {net.hidden}x 32-32, 32-2", "params": net.num_params}
     res["mfma"] = cnn_mfma(net, npairs, 68, 120)
     return res
