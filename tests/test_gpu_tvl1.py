@@ -289,3 +289,21 @@ def test_sigma_too_large_reported():
     with pytest.raises(OpticalFlowError) as e:
         TV_L1(4, 40, nscales=1)
     assert e.value.status == 2
+
+
+@pytest.mark.parametrize("w,h,kw", [
+    (16, 12, {}),                                  # one scale by the nscales rule
+    (300, 48, {}),                                 # 5 scales down to 19x3, thin image
+    (48, 300, {}),                                 # the same, transposed
+    (65, 33, {}),                                  # odd sizes, 3 scales
+    (96, 64, dict(nscales=1, warps=1)),            # one scale, one warping
+    (130, 70, dict(zfactor=0.75, nscales=6, warps=2)),
+])
+def test_tvl1_edge_shapes_match_oracle(w, h, kw):
+    """Thin, tiny and odd frames, and single-scale / single-warping solves:
+    same flow and iteration counts as the oracle."""
+    I0, I1, _, _ = synth.make_pair(w, h, 2.0, 50 + w + h, bumps=False)
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    u, it, uo, ito = _solve_both(q0, q1, **kw)
+    assert np.array_equal(it, ito), (it, ito)
+    assert np.array_equal(u, uo)
