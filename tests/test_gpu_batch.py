@@ -146,3 +146,16 @@ def test_batch_autotune_installs_a_candidate_and_stays_exact():
         uo, ito = O.tvl1(q0, q1)
         assert it[k] == int(ito.sum()), k
         assert np.array_equal(flows[k], uo), k
+
+
+def test_lockstep_edge_shapes_match_oracle():
+    """Tiny, thin and odd frames (1, 3 and 5 scales) in one lockstep batch."""
+    from amdflow.batch import TVL1Batch
+    sizes = [(16, 12), (300, 48), (48, 300), (65, 33), (16, 12), (97, 61)]
+    frames, d0, d1 = _frames(sizes, 2.0, 900)
+    with TVL1Batch(streams=3) as b:
+        it, flows = _run(b, sizes, d0, d1)
+    for k, (q0, q1) in enumerate(frames):
+        uo, ito = O.tvl1(q0, q1)
+        assert it[k] == int(ito.sum()), k
+        assert np.array_equal(flows[k], uo), k
