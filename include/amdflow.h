@@ -395,6 +395,9 @@ int of_lk_compute(of_lk *h, float *Vx, float *Vy, int iframe);
 int of_lk_pyr_compute(const float *I0, const float *I1, int width, int height, int levels,
                       float zf, float sigma, int window, int iters, float det_eps, float *u,
                       float *v);
+/* The same on device pointers, on `stream` (a hipStream_t; NULL = the null
+ * stream); returns when du, dv hold the flow.  The frames are read in place;
+ * du and dv may overlap the frames (then the result is copied out last). */
 int of_lk_pyr_compute_device(const float *dI0, const float *dI1, int width, int height,
                              int levels, float zf, float sigma, int window, int iters,
                              float det_eps, float *du, float *dv, void *stream);
