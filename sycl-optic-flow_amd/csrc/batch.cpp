@@ -706,6 +706,9 @@ int of_tvl1_batch_set_shape(of_tvl1_batch* b, long long min_pixels, int iters_pe
                             int rows_per_wave)
 {
     if (!b) return OF_ERR_ARG;
+    // the batch kernels run the 64-column tiles only
+    if (rows_per_wave == 0 || !amdflow::iter_config_ok(iters_per_launch, rows_per_wave))
+        return OF_ERR_UNSUPPORTED;
     try {
         b->rules.emplace_back(min_pixels, iters_per_launch, rows_per_wave);
     } catch (...) {

@@ -31,8 +31,7 @@ namespace {
 // persistent launch.
 const LaunchRule kDefaultRules[] = {
     {200000, 2, 8, 0},
-    {20000, 4, 4, 1},
-    {0, 4, 4, 1},
+    {0, 4, 0, 0},   // square tiles (rows_per_wave 0), host-paced
 };
 
 constexpr int kEnergyBlocks = 1024;
@@ -64,7 +63,7 @@ int pick_rule(const of_tvl1* h, long long npx, int* K, int* R, int* persistent)
     if (!best)
         for (const LaunchRule& r : kDefaultRules)
             if (npx >= r.min_pixels) { best = &r; break; }
-    if (!best) best = &kDefaultRules[2];
+    if (!best) best = &kDefaultRules[sizeof(kDefaultRules) / sizeof(kDefaultRules[0]) - 1];
     *K = best->K;
     *R = best->R;
     *persistent = best->persistent;
@@ -386,6 +385,8 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
     static const int kShapes[][2] = {{1, 2}, {2, 2}, {1, 4}, {2, 4}, {4, 4},
                                      {1, 8}, {2, 8}, {4, 8}, {8, 8}};
     constexpr int nshape = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+    static const int kSquareK[] = {1, 2, 4};   // square tiles, host-paced
+    constexpr int nsquare = (int)(sizeof(kSquareK) / sizeof(kSquareK[0]));
     if (reps < 1) reps = 1;
     if (h->scale_ev.empty()) {
         h->scale_ev.resize(2 * kMaxScales);
@@ -396,10 +397,13 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
     std::vector<double> best(h->ns, 1e30);
     std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8), bestP(h->ns, 0);
     const std::vector<LaunchRule> saved = h->rules;
-    for (int mode = 0; mode < 2; mode++) {
-        for (int c = 0; c < nshape; c++) {
+    for (int mode = 0; mode < 3; mode++) {
+        for (int c = 0; c < (mode < 2 ? nshape : nsquare); c++) {
             const int row = mode * nshape + c;
-            h->rules.assign(1, LaunchRule{0, kShapes[c][0], kShapes[c][1], mode});
+            const int K = mode < 2 ? kShapes[c][0] : kSquareK[c];
+            const int R = mode < 2 ? kShapes[c][1] : 0;
+            const int P = mode == 1 ? 1 : 0;
+            h->rules.assign(1, LaunchRule{0, K, R, P});
             int st = solve(h, d0, d1, true, h->dflow, hs);   // warm-up of this variant
             // a persistent variant the device cannot run (no cooperative
             // launch) or whose grid barrier ran past its bound is skipped;
@@ -422,9 +426,9 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
                 if (table) table[(size_t)row * h->ns + s] = ms;
                 if (ms < best[s]) {
                     best[s] = ms;
-                    bestK[s] = kShapes[c][0];
-                    bestR[s] = kShapes[c][1];
-                    bestP[s] = mode;
+                    bestK[s] = K;
+                    bestR[s] = R;
+                    bestP[s] = P;
                 }
             }
         }
@@ -515,7 +519,7 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
     
         const size_t N0 = (size_t)width * height;
         size_t maxpart = 1;
-        const int Ks[] = {1, 2, 4, 8}, Rs[] = {2, 4, 8};
+        const int Ks[] = {1, 2, 4, 8}, Rs[] = {0, 2, 4, 8};
         for (int K : Ks)
             for (int R : Rs)
                 if (iter_config_ok(K, R)) {
@@ -868,6 +872,7 @@ int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_laun
         }
         if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
         const int p = persistent ? 1 : 0;
+        if (p && rows_per_wave == 0) return OF_ERR_UNSUPPORTED;   // square tiles: host-paced only
         for (LaunchRule& r : h->rules)
             if (r.min_pixels == min_pixels) {
                 r.K = iters_per_launch;

@@ -135,7 +135,7 @@ def _random_state(rng, h, w):
 
 
 @pytest.mark.parametrize("K,R", [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8),
-                                 (4, 8), (8, 8)])
+                                 (4, 8), (8, 8), (1, 0), (2, 0), (4, 0)])
 @pytest.mark.parametrize("h,w", [(45, 70), (130, 257)])
 def test_iterations_bitexact(K, R, h, w):
     rng = np.random.default_rng(K * 100 + R + h)
@@ -207,6 +207,22 @@ def test_tvl1_iteration_cap_reached():
 @pytest.mark.parametrize("K,R", [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8),
                                  (8, 8)])
 def test_tvl1_launch_shapes_agree(K, R, persistent):
+    _shapes_agree(K, R, persistent)
+
+
+@pytest.mark.parametrize("K", [1, 2, 4])
+def test_tvl1_square_tiles_agree(K):
+    """Square 16x16 tiles (rows_per_wave 0), host-paced, at every scale."""
+    _shapes_agree(K, 0, False)
+
+
+def test_square_tiles_persistent_rejected():
+    with TV_L1(64, 48) as tv:
+        with pytest.raises(Exception):
+            tv.set_launch_rule(0, 4, 0, True)
+
+
+def _shapes_agree(K, R, persistent):
     I0, I1, _, _ = synth.make_pair(240, 180, 5.0, 13, bumps=True)
     q0, q1 = synth.quantize(I0), synth.quantize(I1)
     with TV_L1(240, 180) as tv:
