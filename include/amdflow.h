@@ -122,8 +122,8 @@ int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
  * launch on tiles of 64 x (4*rows_per_wave) pixels with a halo of
  * iters_per_launch pixels.  Supported: iters_per_launch in {1,2,4,8},
  * rows_per_wave in {2,4,8}; rows_per_wave 0 selects square 16 x 16 tiles,
- * one pixel per lane (iters_per_launch in {1,2,4}, host-paced only: the
- * built-in choice below 200000 pixels).  `persistent` = 1 runs each warping of those
+ * one pixel per lane (iters_per_launch in {1,2,4}; the built-in choice,
+ * host-paced, below 200000 pixels).  `persistent` = 1 runs each warping of those
  * scales as one cooperative launch (chunks of iters_per_launch iterations
  * separated by grid barriers) instead of host-paced launches.  Rules are
  * matched largest min_pixels first; calling with min_pixels < 0 clears all
@@ -141,13 +141,14 @@ int of_tvl1_set_inflight(of_tvl1 *h, int launches);
  * over `reps` solves of the device pair dI0/dI1 (width*height bytes each)
  * and installs the fastest variant per scale as launch rules.  `chosen`
  * (optional) receives 3*nscales ints: K, R, persistent for each scale,
- * finest first; `ms_table` (optional) receives OF_TVL1_TUNE_VARIANTS (21)
+ * finest first; `ms_table` (optional) receives OF_TVL1_TUNE_VARIANTS (24)
  * x nscales mean times in ms: rows 0-8 host-paced, 9-17 persistent, shapes
  * in the order (1,2) (2,2) (1,4) (2,4) (4,4) (1,8) (2,8) (4,8) (8,8); rows
  * 18-20 square 16 x 16 tiles (rows_per_wave 0) host-paced with 1, 2 and 4
- * iterations per launch; -1 marks a variant the device cannot run.  Results
- * are bit-identical for every variant; only the time changes. */
-#define OF_TVL1_TUNE_VARIANTS 21
+ * iterations per launch, rows 21-23 the same persistent; -1 marks a variant
+ * the device cannot run.  Results are bit-identical for every variant; only
+ * the time changes. */
+#define OF_TVL1_TUNE_VARIANTS 24
 int of_tvl1_autotune(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, int reps, int *chosen,
                      double *ms_table);
 

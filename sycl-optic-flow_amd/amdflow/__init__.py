@@ -283,7 +283,8 @@ class TV_L1:
 
     AUTOTUNE_SHAPES = [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8), (8, 8)]
     AUTOTUNE_VARIANTS = ([(k, r, 0) for k, r in AUTOTUNE_SHAPES] + [(k, r, 1) for k, r in AUTOTUNE_SHAPES]
-                         + [(1, 0, 0), (2, 0, 0), (4, 0, 0)])   # square 16x16 tiles, host-paced
+                         + [(1, 0, 0), (2, 0, 0), (4, 0, 0)]      # square 16x16 tiles, host-paced
+                         + [(1, 0, 1), (2, 0, 1), (4, 0, 1)])     # and persistent
 
     def autotune(self, d_I0: int, d_I1: int, reps: int = 2):
         """Times every launch variant (K, R, persistent) per scale on a device u8

@@ -385,7 +385,7 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
     static const int kShapes[][2] = {{1, 2}, {2, 2}, {1, 4}, {2, 4}, {4, 4},
                                      {1, 8}, {2, 8}, {4, 8}, {8, 8}};
     constexpr int nshape = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
-    static const int kSquareK[] = {1, 2, 4};   // square tiles, host-paced
+    static const int kSquareK[] = {1, 2, 4};   // square tiles
     constexpr int nsquare = (int)(sizeof(kSquareK) / sizeof(kSquareK[0]));
     if (reps < 1) reps = 1;
     if (h->scale_ev.empty()) {
@@ -397,18 +397,20 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
     std::vector<double> best(h->ns, 1e30);
     std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8), bestP(h->ns, 0);
     const std::vector<LaunchRule> saved = h->rules;
-    for (int mode = 0; mode < 3; mode++) {
+    // modes: 64-column tiles host-paced, persistent; square tiles host-paced,
+    // persistent
+    for (int mode = 0; mode < 4; mode++) {
         for (int c = 0; c < (mode < 2 ? nshape : nsquare); c++) {
-            const int row = mode * nshape + c;
+            const int row = mode < 2 ? mode * nshape + c : 2 * nshape + (mode - 2) * nsquare + c;
             const int K = mode < 2 ? kShapes[c][0] : kSquareK[c];
             const int R = mode < 2 ? kShapes[c][1] : 0;
-            const int P = mode == 1 ? 1 : 0;
+            const int P = (mode == 1 || mode == 3) ? 1 : 0;
             h->rules.assign(1, LaunchRule{0, K, R, P});
             int st = solve(h, d0, d1, true, h->dflow, hs);   // warm-up of this variant
             // a persistent variant the device cannot run (no cooperative
             // launch) or whose grid barrier ran past its bound is skipped;
             // the kernels exit in both cases and the device stays usable
-            if (mode == 1 && st != OF_OK && hipDeviceSynchronize() == hipSuccess) {
+            if (P && st != OF_OK && hipDeviceSynchronize() == hipSuccess) {
                 if (table)
                     for (int s = 0; s < h->ns; s++) table[(size_t)row * h->ns + s] = -1.0;
                 continue;
@@ -872,7 +874,6 @@ int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_laun
         }
         if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
         const int p = persistent ? 1 : 0;
-        if (p && rows_per_wave == 0) return OF_ERR_UNSUPPORTED;   // square tiles: host-paced only
         for (LaunchRule& r : h->rules)
             if (r.min_pixels == min_pixels) {
                 r.K = iters_per_launch;

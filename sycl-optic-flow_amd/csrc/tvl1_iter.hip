@@ -257,7 +257,7 @@ struct SqShared {
 // term over the central tile to acc[t].  Same per-pixel expressions as
 // tile_iterate_impl; pixels outside the image never feed pixels inside (the
 // border rules do not read across the border).
-template <int K>
+template <int K, bool COH>
 __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int by, int cur, int kk,
                                                 double (&acc)[K], SqShared& sh)
 {
@@ -274,12 +274,12 @@ __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int b
     float u1 = 0.0f, u2 = 0.0f, p11 = 0.0f, p12 = 0.0f, p21 = 0.0f, p22 = 0.0f;
     float gx = 0.0f, gy = 0.0f, rc = 0.0f;
     if (in) {
-        u1 = uin[idx];
-        u2 = uin[plane + idx];
-        p11 = pin[idx];
-        p12 = pin[plane + idx];
-        p21 = pin[2 * plane + idx];
-        p22 = pin[3 * plane + idx];
+        u1 = ld<COH>(uin + idx);
+        u2 = ld<COH>(uin + plane + idx);
+        p11 = ld<COH>(pin + idx);
+        p12 = ld<COH>(pin + plane + idx);
+        p21 = ld<COH>(pin + 2 * plane + idx);
+        p22 = ld<COH>(pin + 3 * plane + idx);
         gx = a.wx[idx];
         gy = a.wy[idx];
         rc = a.rhoc[idx];
@@ -337,12 +337,12 @@ __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int b
     if (mine) {
         float* uout = a.u[cur ^ 1];
         float* pout = a.p[cur ^ 1];
-        uout[idx] = u1;
-        uout[plane + idx] = u2;
-        pout[idx] = p11;
-        pout[plane + idx] = p12;
-        pout[2 * plane + idx] = p21;
-        pout[3 * plane + idx] = p22;
+        st<COH>(uout + idx, u1);
+        st<COH>(uout + plane + idx, u2);
+        st<COH>(pout + idx, p11);
+        st<COH>(pout + plane + idx, p12);
+        st<COH>(pout + 2 * plane + idx, p21);
+        st<COH>(pout + 3 * plane + idx, p22);
     }
 }
 
@@ -467,7 +467,7 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_iter(IterArgs a)
     double acc[K];
 #pragma unroll
     for (int t = 0; t < K; t++) acc[t] = 0.0;
-    if constexpr (SQ) tile_iterate_sq<K>(a, blockIdx.x, blockIdx.y, cur, kk, acc, sh_sq);
+    if constexpr (SQ) tile_iterate_sq<K, false>(a, blockIdx.x, blockIdx.y, cur, kk, acc, sh_sq);
     else tile_iterate<K, R, false, NW>(a, blockIdx.x, blockIdx.y, cur, kk, acc, sh);
 
     const unsigned nblocks = gridDim.x * gridDim.y;
@@ -582,10 +582,11 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
 // with device-scope atomic accesses so that halos written by workgroups on
 // other XCDs are seen without cache maintenance.  Partials are
 // double-buffered by chunk parity.
-template <int K, int R, int NW = 4>
+template <int K, int R, int NW = 4, bool SQ = false>
 __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, unsigned ntiles)
 {
     __shared__ TileSharedN<NW> sh;
+    __shared__ SqShared sh_sq;
     __shared__ double sh_err[NW][K];
     __shared__ double sh_tot[K];
     __shared__ double sh4[K * NW];
@@ -607,7 +608,10 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
 #pragma unroll
         for (int t = 0; t < K; t++) acc[t] = 0.0;
         for (unsigned tile = blockIdx.x; tile < ntiles; tile += nb)
-            tile_iterate<K, R, true, NW>(a, (int)(tile % tiles_x), (int)(tile / tiles_x), cur, kk, acc, sh);
+            if constexpr (SQ)
+                tile_iterate_sq<K, true>(a, (int)(tile % tiles_x), (int)(tile / tiles_x), cur, kk, acc, sh_sq);
+            else
+                tile_iterate<K, R, true, NW>(a, (int)(tile % tiles_x), (int)(tile / tiles_x), cur, kk, acc, sh);
         if (redo) {   // the re-run chunk is final: no decision needed
             n += redo;
             cur ^= 1;
@@ -867,6 +871,21 @@ int launch_iter(int K, int R, const IterArgs& a, hipStream_t s)
     return OF_OK;
 }
 
+// The persistent warping kernel of launch shape (K, R): 64 x 4R tiles as
+// kIterWaves waves, or (R = 0) square tiles as 4 waves.
+template <int K, int R>
+static const void* persist_kernel()
+{
+    if constexpr (R == 0) return (const void*)k_tvl1_persist<K, 0, 4, true>;
+    else return (const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>;
+}
+
+template <int R>
+static constexpr int persist_threads()
+{
+    return R == 0 ? 256 : 64 * kIterWaves;
+}
+
 template <int K, int R>
 static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
 {
@@ -876,9 +895,8 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>,
-                64 * kIterWaves, 0) != hipSuccess) {
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel<K, R>(),
+                                                         persist_threads<R>(), 0) != hipSuccess) {
             (void)hipGetLastError();
             return OF_ERR_HIP;
         }
@@ -897,9 +915,8 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
     IterArgs args = a;
     unsigned tx = gx, nt = ntiles;
     void* kargs[] = {(void*)&args, (void*)&tx, (void*)&nt};
-    const hipError_t e =
-        hipLaunchCooperativeKernel((const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>,
-                                   dim3(nb), dim3(64 * kIterWaves), kargs, 0, s);
+    const hipError_t e = hipLaunchCooperativeKernel(persist_kernel<K, R>(), dim3(nb),
+                                                    dim3(persist_threads<R>()), kargs, 0, s);
     if (e != hipSuccess) {
         record_hip_error(e, "k_tvl1_persist cooperative launch", __FILE__, __LINE__);
         return OF_ERR_HIP;
@@ -911,6 +928,7 @@ int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s)
 {
 #define OF_PCASE(k, r) \
     if (K == k && R == r) return launch_persist_kr<k, r>(a, nblocks, s);
+    OF_PCASE(1, 0) OF_PCASE(2, 0) OF_PCASE(4, 0)
     OF_PCASE(1, 2) OF_PCASE(2, 2)
     OF_PCASE(1, 4) OF_PCASE(2, 4) OF_PCASE(4, 4)
     OF_PCASE(1, 8) OF_PCASE(2, 8) OF_PCASE(4, 8) OF_PCASE(8, 8)

@@ -210,16 +210,27 @@ def test_tvl1_launch_shapes_agree(K, R, persistent):
     _shapes_agree(K, R, persistent)
 
 
+@pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("K", [1, 2, 4])
-def test_tvl1_square_tiles_agree(K):
-    """Square 16x16 tiles (rows_per_wave 0), host-paced, at every scale."""
-    _shapes_agree(K, 0, False)
+def test_tvl1_square_tiles_agree(K, persistent):
+    """Square 16x16 tiles (rows_per_wave 0) at every scale."""
+    _shapes_agree(K, 0, persistent)
 
 
-def test_square_tiles_persistent_rejected():
-    with TV_L1(64, 48) as tv:
-        with pytest.raises(Exception):
-            tv.set_launch_rule(0, 4, 0, True)
+@pytest.mark.parametrize("K", [2, 4])
+def test_tvl1_square_persistent_1080p(K):
+    """Persistent square tiles at every scale of a 1920x1080 pair: one
+    workgroup per tile is more than the device holds at the finest scale, so
+    workgroups take several tiles per chunk."""
+    I0, I1, _, _ = synth.make_pair(1920, 1080, 20.0, 2024, bumps=True)
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    with TV_L1(1920, 1080, nscales=5) as tv:
+        tv.set_launch_rule(0, K, 0, True)
+        tv.runDualTVL1Multiscale(q0, q1)
+        u, it = tv.getU(), tv.iterations()
+    uo, ito = O.tvl1(q0, q1, O.default_params(nscales=5))
+    assert np.array_equal(it, ito)
+    assert np.array_equal(u, uo)
 
 
 def _shapes_agree(K, R, persistent):
