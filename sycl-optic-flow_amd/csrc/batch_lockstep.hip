@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256) void k_lp_final(const LsPrep* __restrict__ pre
         ctl->launches = 0;
         ctl->wlaunches = 0;
         ctl->bar_count = 0;
+        for (int g = 0; g < kBarGroups; g++) ctl->grp_count[g].v = 0;
         ctl->error = 0;
     }
 }

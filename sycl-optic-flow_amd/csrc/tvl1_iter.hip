@@ -574,6 +574,76 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
     return *sh_ok != 0;
 }
 
+// Two-level grid barrier, for grids of many workgroups (the square-tile
+// warping kernel: one workgroup per 8x8 centre).  Workgroup b arrives on the
+// counter of group b % 8; the last arrival of a group arrives on the top
+// counter; the last top arrival releases the group leaders, each leader its
+// group.  With all workgroups arriving together a flat counter took ~10 us
+// per barrier at 768 workgroups, the two levels ~3 us.  Each last arriver
+// resets its counter and waits for the reset before it releases anyone; the
+// release words only grow, and `ep` holds their values read at kernel start
+// plus the barriers passed.  Bounded as grid_barrier.
+struct BarEpoch {
+    unsigned top, grp;
+};
+
+__device__ __forceinline__ bool bar_wait(const unsigned* w, unsigned target, TvCtl* ctl)
+{
+    unsigned spins = 0;
+    while ((int)(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1u << 21)) {
+            __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        if ((spins & 1023u) == 0 &&
+            __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool grid_barrier2(TvCtl* ctl, unsigned nb, BarEpoch& ep, int* sh_ok)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned g = blockIdx.x % kBarGroups;
+        const unsigned members = nb / kBarGroups + (g < nb % kBarGroups ? 1u : 0u);
+        const unsigned groups = nb < (unsigned)kBarGroups ? nb : (unsigned)kBarGroups;
+        unsigned* gc = &ctl->grp_count[g].v;
+        unsigned* gg = &ctl->grp_gen[g].v;
+        int ok = 1;
+        const unsigned old = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == members - 1) {
+            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned t =
+                __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t == groups - 1) {
+                __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __hip_atomic_store(&ctl->bar_gen, ep.top + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                ok = bar_wait(&ctl->bar_gen, ep.top + 1, ctl);
+            }
+            if (ok) {
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __hip_atomic_store(gg, ep.grp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            ok = bar_wait(gg, ep.grp + 1, ctl);
+        }
+        ep.top++;
+        ep.grp++;
+        *sh_ok = ok;
+    }
+    __syncthreads();
+    return *sh_ok != 0;
+}
+
 // A whole warping: chunks of K iterations over all tiles (workgroup b takes
 // tiles b, b+nb, ...), a grid barrier, then every workgroup sums the chunk's
 // per-workgroup error partials in the same fixed order and takes the same
@@ -595,7 +665,13 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
     TvCtl* ctl = a.ctl;
     const unsigned nb = gridDim.x;
     unsigned gen = 0;
-    if (threadIdx.x == 0) gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    BarEpoch ep{0u, 0u};
+    if (threadIdx.x == 0) {
+        gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ep.top = gen;
+        ep.grp = __hip_atomic_load(&ctl->grp_gen[blockIdx.x % kBarGroups].v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+    }
     int cur = ctl->cur[a.level];
     int n = 0;
     int redo = 0;
@@ -622,7 +698,8 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
         if (threadIdx.x < (unsigned)kk)
             __hip_atomic_store(pp + (size_t)threadIdx.x * nb + blockIdx.x, part, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        if (!grid_barrier(ctl, nb, gen, &sh_ok)) return;
+        // square tiles run one workgroup per small tile: many workgroups
+        if (!(SQ ? grid_barrier2(ctl, nb, ep, &sh_ok) : grid_barrier(ctl, nb, gen, &sh_ok))) return;
         grid_sums<K, NW>(pp, nb, kk, sh4, sh_tot);
         const int keep = stop_after(sh_tot, kk, n, a);
         if (a.err_log && blockIdx.x == 0 && threadIdx.x < (unsigned)kk)

@@ -28,6 +28,7 @@ __global__ void k_ctl_init(TvCtl* ctl)
         ctl->maxkey[0] = ctl->maxkey[1] = 0ull;
         ctl->minkey[0] = ctl->minkey[1] = ~0ull;
         ctl->bar_count = 0;
+        for (int g = 0; g < kBarGroups; g++) ctl->grp_count[g].v = 0;
         ctl->error = 0;
     }
 }

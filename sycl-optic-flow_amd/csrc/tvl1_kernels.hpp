@@ -10,6 +10,12 @@ namespace amdflow {
 
 constexpr int kMaxScales = 32;
 constexpr int kMinMaxBlocks = 1024;   // grid of the frame conversion + min/max kernel
+// Workgroup groups of the two-level grid barrier: workgroup b belongs to
+// group b % 8, the XCD the dispatcher places it on.
+constexpr int kBarGroups = 8;
+struct alignas(256) BarWord {
+    unsigned v;
+};
 
 // Device control block of one solve.  Written by the last workgroup of each
 // iteration launch (and by the set-up kernels), read by every workgroup at
@@ -30,6 +36,8 @@ struct TvCtl {
     // their own, away from each other and from the fields above
     alignas(256) unsigned bar_count;
     alignas(256) unsigned bar_gen;
+    BarWord grp_count[kBarGroups];   // two-level barrier: per-group arrivals
+    BarWord grp_gen[kBarGroups];     // and per-group release epochs
     alignas(256) int error;         // set if a barrier wait ran past its bound
 };
 
