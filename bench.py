@@ -31,7 +31,7 @@ HBM_PEAK_GBPS = 8000.0
 # --copy-too); the runs are summarised under profiles/.  Counters cannot be
 # read inside a timed run, so the value is carried here and reported only
 # when the same kernel dominates.
-MEASURED_TRAFFIC = {"k_tvl1_persist<2,8>": 1778589711}
+MEASURED_TRAFFIC = {"k_tvl1_persist<2,8>": 1777462856}
 WORKLOAD = "tvl1_1920x1080"
 SEED = 2024
 
