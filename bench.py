@@ -478,6 +478,10 @@ def main():
         "launch_shapes_source": "autotuned in this run" if tuned_now else "tuning table",
         "roofline": roofline,
     }
+    # the solver handle is done; closing it also releases the stream the
+    # autotuner gave it, which would otherwise share a hardware queue with a
+    # batch lane
+    tv.close()
     if args.batch_pairs > 0:
         result["batched"] = batched(args.batch_pairs, args.batch_lanes, reps=args.batch_reps,
                                     workers=not args.no_batch_workers)
