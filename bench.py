@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--batch-lanes", type=int, default=3,
                     help="lanes (concurrent streams) of the lockstep batch solver")
     ap.add_argument("--batch-reps", type=int, default=3, help="timed runs of the batch")
+    ap.add_argument("--batch-shapes", default=None, metavar="K,R,K,R",
+                    help="fixed lockstep launch shapes (iterations per launch, rows per wave) at "
+                         "scales of >= 200000 pixels and below, instead of the batch autotuner")
     ap.add_argument("--no-batch-workers", action="store_true",
                     help="skip the worker-mode comparison of the batch")
     ap.add_argument("--no-other", action="store_true", help="skip the HS and LK measurements")
@@ -80,7 +83,7 @@ def calibrate_copy():
 BATCH_SIZES = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
 
 
-def batched(npairs: int, lanes: int, reps: int, workers: bool = True):
+def batched(npairs: int, lanes: int, reps: int, workers: bool = True, shapes=None):
     """64 pairs whose sizes are drawn uniformly (seeded) from BATCH_SIZES,
     same generator and parameters as the 1920x1080 workload, solved by the
     lockstep batch solver with launch shapes from its autotuner (and, for
@@ -122,10 +125,17 @@ def batched(npairs: int, lanes: int, reps: int, workers: bool = True):
 
     def timed(mode, streams):
         with TVL1Batch(streams=streams, nscales=5, mode=mode) as b:
-            if mode == "lockstep":
-                # launch shapes from the batch autotuner (untimed)
-                shapes, table, best = b.autotune(ws, hs, d0, d1, up, reps=2)
+            if mode == "lockstep" and shapes is not None:
+                # fixed launch shapes (profiler runs: serialised dispatches
+                # would skew the autotuner's timings)
+                b.set_shape(200000, *shapes[0])
+                b.set_shape(0, *shapes[1])
                 tune.update(shapes={"large_scales": list(shapes[0]), "small_scales": list(shapes[1]),
+                                    "split_pixels": 200000}, source="--batch-shapes")
+            elif mode == "lockstep":
+                # launch shapes from the batch autotuner (untimed)
+                tuned, table, best = b.autotune(ws, hs, d0, d1, up, reps=2)
+                tune.update(shapes={"large_scales": list(tuned[0]), "small_scales": list(tuned[1]),
                                     "split_pixels": 200000},
                             uniform_ms={f"{k}x{r}": round(v, 2) for (k, r), v in table.items()},
                             chosen_ms=round(best, 2))
@@ -377,6 +387,12 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(pairs are independent: N GPUs would each solve their own share)", file=sys.stderr)
         sys.exit(2)
+    bshapes = None
+    if args.batch_shapes:
+        v = [int(x) for x in args.batch_shapes.split(",")]
+        if len(v) != 4:
+            sys.exit("--batch-shapes takes K,R,K,R (large scales, then small scales)")
+        bshapes = ((v[0], v[1]), (v[2], v[3]))
     if args.calibrate_copy or args.copy_too:
         calibrate_copy()
         if args.calibrate_copy:
@@ -484,7 +500,7 @@ def main():
     tv.close()
     if args.batch_pairs > 0:
         result["batched"] = batched(args.batch_pairs, args.batch_lanes, reps=args.batch_reps,
-                                    workers=not args.no_batch_workers)
+                                    workers=not args.no_batch_workers, shapes=bshapes)
     if args.warm_pairs > 0:
         result["warm_start"] = warm_start(args.warm_pairs)
     if not args.no_other:
