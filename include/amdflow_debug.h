@@ -10,6 +10,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "amdflow.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -39,6 +41,23 @@ int of_dbg_tvl1_iterations(const float *I1wx, const float *I1wy, const float *rh
                            int iters_per_launch, int rows_per_wave, double *err_sums);
 /* hypot as the solver evaluates it on the GPU, element-wise. */
 int of_dbg_hypot(const float *a, const float *b, float *out, size_t n);
+
+/* One scale of the solver, dualTVL1(I0, I1, u1, u2, nx, ny)
+ * (tvl1.cpp:182-294), on float frames taken as they are (no normalisation,
+ * pre-smoothing or pyramid): centred gradient of I1, p = 0, then
+ * params->warps warpings of warp set-up and iterations under the stopping
+ * rule, with the product's launch shapes.  u1, u2 hold the initial flow and
+ * receive the result; iters receives the params->warps iteration counts.
+ * params->nscales is ignored. */
+int of_dbg_tvl1_dual(const float *I0, const float *I1, float *u1, float *u2, int width,
+                     int height, const of_tvl1_params *params, int *iters);
+
+/* Horn-Schunck pyramid steps on dense planes (stride = width): the 2x2 box
+ * downscale of a w x h plane to nw x nh with 2*nw <= w, 2*nh <= h
+ * (downscaleKernel.cuh:27-40), and the x2 flow upscale of a w x h field to
+ * nw x nh times `scale` (upscaleKernel.cuh:26-40). */
+int of_dbg_hs_downscale(const float *src, int w, int h, float *out, int nw, int nh);
+int of_dbg_hs_upscale(const float *src, int w, int h, float *out, int nw, int nh, float scale);
 
 /* Horn-Schunck: `iters` Jacobi sweeps (solverKernel.cuh:145-152) from
  * (du, dv) with launch shape (sweeps_per_launch K, rows_per_wave R). */

@@ -428,6 +428,23 @@ void orc_tvl1_warp_setup(const float *I0, const float *I1, const float *I1x, con
         rho_c[i] = ((I1w[i] - I1wx[i] * u1[i]) - I1wy[i] * u2[i]) - I0[i];
 }
 
+/* How the stopping error of a warping is accumulated.
+ *   ORC_ERROR_DOUBLE_ROWS (default): the float per-pixel terms summed in
+ *     double, row by row, rows added in order -- independent of the thread
+ *     count, and the order the GPU's fixed-order sum reproduces bit for bit.
+ *   ORC_ERROR_FLOAT_SEQUENTIAL: the reference's `float error` accumulated
+ *     pixel by pixel in index order and divided in float (tvl1.cpp:252-265
+ *     as it runs with one OpenMP thread; with more threads the reference's
+ *     order depends on the thread count and completion order).
+ * The two can only disagree on a stopping decision when error/N lies within
+ * the float sum's rounding of epsilon^2. */
+static int g_error_mode = ORC_ERROR_DOUBLE_ROWS;
+
+void orc_set_error_mode(int mode)
+{
+    g_error_mode = mode == ORC_ERROR_FLOAT_SEQUENTIAL ? mode : ORC_ERROR_DOUBLE_ROWS;
+}
+
 /* One primal-dual iteration from a given state (tvl1.cpp:260-333).  Updates
  * u1,u2,p11..p22 in place and returns sum over pixels of
  * (du1^2 + du2^2) accumulated in double (each pixel term in float). */
@@ -442,6 +459,14 @@ double orc_tvl1_iteration(const float *I1wx, const float *I1wy, const float *rho
     float *v1 = scratch, *v2 = scratch + size;
     float *div1 = scratch + 2 * size, *div2 = scratch + 3 * size;
     double *rows = (double *)malloc(sizeof(double) * (size_t)ny);
+    /* u before the update, kept only for the float replay of the error */
+    float *u1k_save = NULL, *u2k_save = NULL;
+    if (g_error_mode == ORC_ERROR_FLOAT_SEQUENTIAL) {
+        u1k_save = (float *)malloc(sizeof(float) * 2 * size);
+        u2k_save = u1k_save + size;
+        memcpy(u1k_save, u1, sizeof(float) * size);
+        memcpy(u2k_save, u2, sizeof(float) * size);
+    }
 
     orc_tvl1_th(rho_c, I1wx, I1wy, u1, u2, l_t, v1, v2, size);
     orc_divergence(p11, p12, div1, nx, ny);
@@ -466,6 +491,18 @@ double orc_tvl1_iteration(const float *I1wx, const float *I1wy, const float *rho
     double err = 0.0;
     for (int i = 0; i < ny; i++) err += rows[i];
     free(rows);
+    if (g_error_mode == ORC_ERROR_FLOAT_SEQUENTIAL) {
+        /* the reference's `float error` summed pixel by pixel in index order
+         * (tvl1.cpp:252-264 with one OpenMP thread); the terms are the same
+         * float expressions, recomputed from the new and the saved u */
+        float ef = 0.0f;
+        for (size_t p = 0; p < size; p++) {
+            const float d1 = u1[p] - u1k_save[p], d2 = u2[p] - u2k_save[p];
+            ef += d1 * d1 + d2 * d2;
+        }
+        err = (double)ef;
+    }
+    free(u1k_save);
 
     /* forward gradients go into the div/v scratch as in tvl1.cpp:310-311 */
     orc_forward_gradient(u1, div1, v1, nx, ny);
@@ -507,11 +544,16 @@ int orc_dual_tvl1(const float *I0, const float *I1, float *u1, float *u2, int nx
         orc_tvl1_warp_setup(I0, I1, I1x, I1y, u1, u2, I1w, I1wx, I1wy, rho_c, nx, ny);
         int n = 0;
         double error = INFINITY;
-        while (error > (double)eps2 && n < prm->max_iterations) {
+        float error_f = INFINITY;
+        const int fmode = g_error_mode == ORC_ERROR_FLOAT_SEQUENTIAL;
+        while ((fmode ? error_f > eps2 : error > (double)eps2) && n < prm->max_iterations) {
             n++;
             const double s = orc_tvl1_iteration(I1wx, I1wy, rho_c, u1, u2, p11, p12, p21, p22,
                                                 nx, ny, prm->tau, prm->lambda, prm->theta, scratch);
             error = s / (double)size;
+            /* tvl1.cpp:265: float error divided by the size_t size, in float */
+            error_f = (float)s;
+            error_f /= (float)size;
         }
         if (iters) iters[w] = n;
     }

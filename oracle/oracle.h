@@ -12,6 +12,7 @@ extern "C" {
 #endif
 
 enum { ORC_OK = 0, ORC_ERR_ARG = 1, ORC_ERR_SIGMA = 2, ORC_ERR_ALLOC = 3 };
+enum { ORC_ERROR_DOUBLE_ROWS = 0, ORC_ERROR_FLOAT_SEQUENTIAL = 1 };
 
 typedef struct {
     float tau, lambda, theta;
@@ -26,6 +27,8 @@ typedef struct {
 void orc_set_threads(int n);
 int orc_get_threads(void);
 void orc_tvl1_default_params(orc_tvl1_params *p);
+/* Accumulation of the TV-L1 stopping error (ORC_ERROR_*; process-wide). */
+void orc_set_error_mode(int mode);
 int orc_tvl1_nscales(int width, int height, int nscales, float zfactor);
 void orc_zoom_size(int nx, int ny, float factor, int *nxx, int *nyy);
 float orc_zoom_sigma(float factor);

@@ -130,3 +130,43 @@ def hs_warp_derivatives(I0, I1, u, v):
     _check(_lib.of_dbg_hs_warp_derivatives(_p(I0), _p(I1), _p(u), _p(v), w, h,
                                            *[_p(o) for o in outs]), "dbg_hs_warp_derivatives")
     return tuple(outs)  # I1w, Ix, Iy, Iz
+
+
+_lib.of_dbg_hs_downscale.restype = C.c_int
+_lib.of_dbg_hs_downscale.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int]
+_lib.of_dbg_hs_upscale.restype = C.c_int
+_lib.of_dbg_hs_upscale.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_float]
+
+
+def hs_downscale(src, nw, nh):
+    """2x2 box average of the HS pyramid (dense planes)."""
+    a = _f32(src)
+    h, w = a.shape
+    out = np.empty((nh, nw), np.float32)
+    _check(_lib.of_dbg_hs_downscale(_p(a), w, h, _p(out), nw, nh), "dbg_hs_downscale")
+    return out
+
+
+def hs_upscale(src, nw, nh, scale):
+    """x2 flow upscale of the HS pyramid times `scale` (dense planes)."""
+    a = _f32(src)
+    h, w = a.shape
+    out = np.empty((nh, nw), np.float32)
+    _check(_lib.of_dbg_hs_upscale(_p(a), w, h, _p(out), nw, nh, scale), "dbg_hs_upscale")
+    return out
+
+
+def tvl1_dual(I0, I1, u1, u2, params):
+    """One scale of the GPU solver (warpings under the stopping rule) on float
+    frames as given; `params` is an amdflow.TVL1Params.  Returns (u1, u2,
+    iteration count per warping)."""
+    from . import TVL1Params
+    I0, I1 = _f32(I0), _f32(I1)
+    a, b = _f32(u1).copy(), _f32(u2).copy()
+    h, w = I0.shape
+    iters = np.zeros(params.warps, np.int32)
+    _lib.of_dbg_tvl1_dual.restype = C.c_int
+    _lib.of_dbg_tvl1_dual.argtypes = [_vp] * 4 + [C.c_int, C.c_int, C.POINTER(TVL1Params), _vp]
+    _check(_lib.of_dbg_tvl1_dual(_p(I0), _p(I1), _p(a), _p(b), w, h, C.byref(params), _p(iters)),
+           "dbg_tvl1_dual")
+    return a, b, iters

@@ -558,4 +558,42 @@ int of_dbg_hs_warp_derivatives(const float* I0, const float* I1, const float* u,
     return st;
 }
 
+// Runs one pyramid step (down: 2x2 box; up: x2 flow upscale times `scale`)
+// on a dense w x h plane into a dense nw x nh plane.
+static int dbg_hs_resample(const float* src, int w, int h, float* out, int nw, int nh, bool up,
+                           float scale)
+{
+    if (!src || !out || w < 1 || h < 1 || nw < 1 || nh < 1) return OF_ERR_ARG;
+    // the box reads rows/columns 2i and 2i+1 of the source
+    if (!up && (2 * (long long)nw > w || 2 * (long long)nh > h)) return OF_ERR_ARG;
+    const size_t n = (size_t)w * h, nn = (size_t)nw * nh;
+    float *ds = nullptr, *dd = nullptr;
+    int st = dev_alloc(&ds, n);
+    if (!st) st = dev_alloc(&dd, nn);
+    if (!st && hipMemcpy(ds, src, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = OF_ERR_HIP;
+    if (!st) {
+        if (up)
+            hipLaunchKernelGGL(k_hs_upscale, grid2d(nw, nh), dim3(256), 0, nullptr, ds, w, h, w, dd,
+                               nw, nh, nw, scale);
+        else
+            hipLaunchKernelGGL(k_hs_downscale, grid2d(nw, nh), dim3(256), 0, nullptr, ds, w, dd, nw,
+                               nh, nw);
+        if (hipGetLastError() != hipSuccess) st = OF_ERR_HIP;
+    }
+    if (!st && hipMemcpy(out, dd, nn * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) st = OF_ERR_HIP;
+    dev_free(ds);
+    dev_free(dd);
+    return st;
+}
+
+int of_dbg_hs_downscale(const float* src, int w, int h, float* out, int nw, int nh)
+{
+    return dbg_hs_resample(src, w, h, out, nw, nh, false, 1.0f);
+}
+
+int of_dbg_hs_upscale(const float* src, int w, int h, float* out, int nw, int nh, float scale)
+{
+    return dbg_hs_resample(src, w, h, out, nw, nh, true, scale);
+}
+
 }  // extern "C"

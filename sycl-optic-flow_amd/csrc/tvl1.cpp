@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "amdflow.h"
+#include "amdflow_debug.h"
 #include "common.hpp"
 #include "tvl1_kernels.hpp"
 #include "tvl1_handle.hpp"
@@ -651,6 +652,44 @@ int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int first_scale, int
         if (warps == 0) warps = h->prm.warps;
         if (max_iterations == 0) max_iterations = h->prm.max_iterations;
         return solve_levels(h, dUinit, first_scale, warps, max_iterations, dU, st);
+    });
+}
+
+int of_dbg_tvl1_dual(const float* I0, const float* I1, float* u1, float* u2, int width, int height,
+                     const of_tvl1_params* params, int* iters)
+{
+    return guarded([&]() -> int {
+        if (!I0 || !I1 || !u1 || !u2 || !params || !iters) return OF_ERR_ARG;
+        of_tvl1_params p = *params;
+        p.nscales = 1;
+        of_tvl1* h = nullptr;
+        OF_TRY(of_tvl1_create(&p, width, height, &h));
+        const size_t n = (size_t)width * height;
+        float* dinit = nullptr;
+        hipStream_t st = nullptr;
+        int rc = own_stream(h, &st);
+        if (!rc) rc = dev_alloc(&dinit, 2 * n);
+        // the frames go straight into scale 0 of the pyramids (off[0] = 0)
+        auto up = [&](float* dst, const float* src) {
+            if (!rc && hipMemcpy(dst, src, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                rc = OF_ERR_HIP;
+        };
+        up(h->I0p, I0);
+        up(h->I1p, I1);
+        up(dinit, u1);
+        up(dinit + n, u2);
+        if (!rc) {
+            h->prepared = true;
+            rc = solve_levels(h, dinit, 0, p.warps, p.max_iterations, h->dflow, st);
+        }
+        if (!rc && hipMemcpy(u1, h->dflow, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = OF_ERR_HIP;
+        if (!rc && hipMemcpy(u2, h->dflow + n, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = OF_ERR_HIP;
+        if (!rc) std::copy(h->h_iters.begin(), h->h_iters.begin() + p.warps, iters);
+        dev_free(dinit);
+        release(h);
+        return rc;
     });
 }
 
