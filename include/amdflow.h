@@ -26,7 +26,7 @@ typedef enum {
     OF_ERR_ARG = 1,          /* invalid size or parameter */
     OF_ERR_SIGMA = 2,        /* Gaussian window larger than an image level
                                 (the reference's "GaussianSmooth: sigma too
-                                large", mask.cpp:231-234) */
+                                large", mask.cpp:218-221) */
     OF_ERR_ALLOC = 3,        /* device or host allocation failed */
     OF_ERR_HIP = 4,          /* a HIP runtime call failed */
     OF_ERR_STATE = 5,        /* call not valid in the handle's state */
@@ -60,9 +60,9 @@ typedef struct {
 void of_tvl1_default_params(of_tvl1_params *p);
 
 /* Constructor: TV_L1::TV_L1(int width, int height, float tau, ...)
- * (src/tvl1/sequential/tvl1.cpp:69-121).  The number of scales follows the
+ * (src/tvl1/sequential/tvl1.cpp:27-79).  The number of scales follows the
  * reference rule min(nscales, 1 + ln(hypot(W,H)/16)/ln(1/zfactor))
- * (tvl1.cpp:84-85), clamped to >= 1.  `params` may be NULL for defaults. */
+ * (tvl1.cpp:42-43), clamped to >= 1.  `params` may be NULL for defaults. */
 int of_tvl1_create(const of_tvl1_params *params, int width, int height, of_tvl1 **out);
 void of_tvl1_destroy(of_tvl1 *h);
 
@@ -71,13 +71,13 @@ int of_tvl1_nscales(const of_tvl1 *h);
 int of_tvl1_scale_size(const of_tvl1 *h, int s, int *nx, int *ny);
 
 /* Pair solve on 8-bit frames: TV_L1::runDualTVL1Multiscale(uint8_t* I0,
- * uint8_t* I1) (src/tvl1/sequential/tvl1.hpp:17, tvl1.cpp:171-217).
+ * uint8_t* I1) (src/tvl1/sequential/tvl1.hpp:17, tvl1.cpp:129-175).
  * Host pointers of width*height bytes each.  Flow available through
  * of_tvl1_flow. */
 int of_tvl1_run_u8(of_tvl1 *h, const uint8_t *I0, const uint8_t *I1);
 
 /* Same solve on float frames (host pointers, width*height floats each);
- * the u8->float cast of tvl1.cpp:160-165 is skipped. */
+ * the u8->float cast of tvl1.cpp:118-123 is skipped. */
 int of_tvl1_run_f32(of_tvl1 *h, const float *I0, const float *I1);
 
 /* Streaming solve: TV_L1::runDualTVL1Multiscale(const float* I) of the GPU
@@ -233,13 +233,13 @@ int of_tvl1_run_batch(const of_tvl1_params *params, int npairs, const int *width
  * ---------------------------------------------------------------------- */
 /* Initial flow for the next solve of this handle (any of_tvl1_run_* or
  * of_tvl1_push_f32 call; no reference counterpart: the reference starts the
- * coarsest scale from zero, tvl1.cpp:199-200).  u and v (host or device
+ * coarsest scale from zero, tvl1.cpp:157-158).  u and v (host or device
  * memory) hold scale `level`'s flow components, of_tvl1_scale_size(level)
  * floats each, in that scale's pixel units.  The next solve starts at
  * `level` from that flow with the handle's warpings and iteration cap and
  * skips the coarser scales; the flow is used once.  u = v = NULL clears it. */
 int of_tvl1_set_init_flow(of_tvl1 *h, int level, const float *u, const float *v);
-/* Phase 1 of runDualTVL1Multiscale (src/tvl1/sequential/tvl1.cpp:139-155):
+/* Phase 1 of runDualTVL1Multiscale (src/tvl1/sequential/tvl1.cpp:137-154):
  * frames to float, joint normalisation, pre-smoothing, both pyramids. */
 int of_tvl1_prepare_device_u8(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, void *stream);
 /* Device pointers to pyramid scale s of the prepared I0 and I1 (row-major,

@@ -8,7 +8,13 @@
 //     void runDualTVL1Multiscale(uint8_t* I0, uint8_t* I1)      pair solve
 //     void runDualTVL1Multiscale(const float* I)                streaming
 //     float* getU()                                             planar [u1|u2]
-//   initFlow / ComputeFlow / deleteFlow_mem   src/hs/cuda/flow.cuh:19-33
+//   void initFlow(int nLevels, int stride, int width, int height)
+//   void ComputeFlow(const float* I0, const float* I1, int width, int height,
+//                    int stride, float alpha, int nLevels, int nWarpIters,
+//                    int nSolverIters, float* u, float* v)
+//   void deleteFlow_mem(int nLevels)               src/hs/cuda/flow.cuh:19-33
+//     (one process-wide state, as in the reference; initFlowHandle and the
+//     of_hs* overloads of the other two are the reentrant form)
 //   LucasKanade(int spac, int temp, int window, int nx, int ny)
 //     void copy_frames_circularbuffer_GPU_wrapper(unsigned char*, int, int, int)
 //     void lucas_kanade(float* Vx, float* Vy, int iframe)  src/lk/cuda/lucaskanade.cuh:6-35
@@ -76,9 +82,18 @@ private:
 };
 
 // Horn-Schunck with the reference's free-function interface.  The reference
-// keeps one global state (flow.cu:26-44); here initFlow returns a handle that
-// the other two calls take.
-inline of_hs* initFlow(int nLevels, int stride, int width, int height)
+// keeps one process-wide state (flow.cu:26-44) that initFlow allocates,
+// ComputeFlow uses and deleteFlow_mem frees; the three functions below with
+// the reference's signatures do the same over one library handle, so they are
+// not reentrant either.  initFlowHandle returns a handle of its own that the
+// of_hs* overloads take: one per thread or per frame size.
+inline of_hs*& hs_global_state()
+{
+    static of_hs* h = nullptr;
+    return h;
+}
+
+inline of_hs* initFlowHandle(int nLevels, int stride, int width, int height)
 {
     of_hs* h = nullptr;
     check(of_hs_create(nLevels, stride, width, height, &h), "initFlow");
@@ -95,6 +110,31 @@ inline void ComputeFlow(of_hs* h, const float* I0, const float* I1, int width, i
 }
 
 inline void deleteFlow_mem(of_hs* h) { of_hs_destroy(h); }
+
+// flow.cuh:19: a second initFlow replaces the state of the first
+inline void initFlow(int nLevels, int stride, int width, int height)
+{
+    of_hs* h = initFlowHandle(nLevels, stride, width, height);
+    of_hs_destroy(hs_global_state());
+    hs_global_state() = h;
+}
+
+// flow.cuh:21-31; OF_ERR_STATE before initFlow
+inline void ComputeFlow(const float* I0, const float* I1, int width, int height, int stride,
+                        float alpha, int nLevels, int nWarpIters, int nSolverIters, float* u,
+                        float* v)
+{
+    if (!hs_global_state()) throw Error(OF_ERR_STATE, "ComputeFlow before initFlow");
+    ComputeFlow(hs_global_state(), I0, I1, width, height, stride, alpha, nLevels, nWarpIters,
+                nSolverIters, u, v);
+}
+
+// flow.cuh:33
+inline void deleteFlow_mem(int /*nLevels*/)
+{
+    of_hs_destroy(hs_global_state());
+    hs_global_state() = nullptr;
+}
 
 class LucasKanade {
 public:

@@ -16,22 +16,22 @@
 extern "C" {
 #endif
 
-/* gaussian(I, xdim, ydim, sigma, buffer), mask.cpp:217-287 (out of place). */
+/* gaussian(I, xdim, ydim, sigma, buffer), mask.cpp:204-274 (out of place). */
 int of_dbg_gaussian(const float *in, float *out, int width, int height, double sigma);
 /* zoom_out(I, Iout, nx, ny, factor, ...), zoom.cpp:20-53. */
 int of_dbg_zoom_out(const float *in, float *out, int width, int height, float factor);
 /* zoom_in(I, Iout, nx, ny, nxx, nyy), zoom.cpp:61-85, times `scale`. */
 int of_dbg_zoom_in(const float *in, float *out, int width, int height, int nxx, int nyy,
                    float scale);
-/* centered_gradient, mask.cpp:152-209. */
+/* centered_gradient, mask.cpp:139-196. */
 int of_dbg_centered_gradient(const float *in, float *dx, float *dy, int width, int height);
-/* image_normalization of two frames, tvl1.cpp:344-371 (in place). */
+/* image_normalization of two frames, tvl1.cpp:302-329 (in place). */
 int of_dbg_normalize(float *I0, float *I1, int width, int height);
-/* warp set-up of one warping, tvl1.cpp:237-254: I1wx, I1wy, rho_c. */
+/* warp set-up of one warping, tvl1.cpp:198-212: I1wx, I1wy, rho_c. */
 int of_dbg_warp_setup(const float *I0, const float *I1, const float *I1x, const float *I1y,
                       const float *u1, const float *u2, float *I1wx, float *I1wy, float *rho_c,
                       int width, int height);
-/* `iters` primal-dual iterations (tvl1.cpp:260-333) from the given state,
+/* `iters` primal-dual iterations (tvl1.cpp:218-291) from the given state,
  * with launch shape (iters_per_launch, rows_per_wave) and no stopping
  * test; state arrays are updated in place; err_sums receives the per
  * iteration sum of (du1^2+du2^2) (double). */

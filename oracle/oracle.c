@@ -58,7 +58,7 @@
 /* pyramid sizes                                                             */
 /* ------------------------------------------------------------------------- */
 
-/* tvl1.cpp:84-85.  N is computed as in the reference: hypot of the integer
+/* tvl1.cpp:42-43.  N is computed as in the reference: hypot of the integer
  * sizes in double, log(1/zfactor) in float (std::log(float) overload), the
  * quotient in double, assigned to float, then min'ed with nscales in float and
  * truncated to int.  Clamped to >= 1 (the reference yields 0 for tiny images,
@@ -82,10 +82,10 @@ void orc_zoom_size(int nx, int ny, float factor, int *nxx, int *nyy)
 }
 
 /* ------------------------------------------------------------------------- */
-/* Gaussian (mask.cpp:217-287)                                               */
+/* Gaussian (mask.cpp:204-274)                                               */
 /* ------------------------------------------------------------------------- */
 
-/* Tap count and normalised taps, mask.cpp:225-244.  Returns the tap count. */
+/* Tap count and normalised taps, mask.cpp:212-231.  Returns the tap count. */
 int orc_gaussian_taps(double sigma, double *B, int max_taps)
 {
     const double den = 2 * sigma * sigma;
@@ -105,9 +105,9 @@ int orc_gaussian_taps(double sigma, double *B, int max_taps)
 }
 
 /* One filtered line.  `line` holds n samples; the window extends with a
- * whole-sample mirror on the low side (x[-k] = x[k], mask.cpp:253-254) and a
- * half-sample mirror on the high side (x[n-1+k] = x[n-k], mask.cpp:255).
- * Accumulation in double, pair sums in float (mask.cpp:260-263). */
+ * whole-sample mirror on the low side (x[-k] = x[k], mask.cpp:240-241) and a
+ * half-sample mirror on the high side (x[n-1+k] = x[n-k], mask.cpp:242).
+ * Accumulation in double, pair sums in float (mask.cpp:247-250). */
 static void gaussian_line(const float *line, float *out, int n, const double *B, int size)
 {
     for (int x = 0; x < n; x++) {
@@ -123,7 +123,7 @@ static void gaussian_line(const float *line, float *out, int n, const double *B,
     }
 }
 
-/* In-place separable Gaussian, rows then columns (mask.cpp:217-287).
+/* In-place separable Gaussian, rows then columns (mask.cpp:204-274).
  * Returns ORC_ERR_SIGMA if the window does not fit the image. */
 int orc_gaussian(float *I, int xdim, int ydim, double sigma)
 {
@@ -272,10 +272,10 @@ void orc_zoom_in(const float *I, float *Iout, int nx, int ny, int nxx, int nyy)
 }
 
 /* ------------------------------------------------------------------------- */
-/* finite differences (mask.cpp:44-209)                                      */
+/* finite differences (mask.cpp:31-196)                                      */
 /* ------------------------------------------------------------------------- */
 
-/* mask.cpp:152-209: centred differences times 0.5 (double literal), with
+/* mask.cpp:139-196: centred differences times 0.5 (double literal), with
  * one-sided differences (still times 0.5) on the border rows/columns. */
 void orc_centered_gradient(const float *in, float *dx, float *dy, int nx, int ny)
 {
@@ -292,7 +292,7 @@ void orc_centered_gradient(const float *in, float *dx, float *dy, int nx, int ny
         }
 }
 
-/* mask.cpp:99-144: forward differences, zero on the last column (x) and on
+/* mask.cpp:86-131: forward differences, zero on the last column (x) and on
  * the last row (y). */
 void orc_forward_gradient(const float *f, float *fx, float *fy, int nx, int ny)
 {
@@ -305,7 +305,7 @@ void orc_forward_gradient(const float *f, float *fx, float *fy, int nx, int ny)
         }
 }
 
-/* Divergence at one pixel with the Chambolle border rules of mask.cpp:44-90.
+/* Divergence at one pixel with the Chambolle border rules of mask.cpp:31-77.
  * The first/last column cases associate left to right exactly as written in
  * the reference (:81-82), which is not the same rounding as dx + dy. */
 static inline float div_at(const float *v1, const float *v2, int i, int j, int nx, int ny)
@@ -314,7 +314,7 @@ static inline float div_at(const float *v1, const float *v2, int i, int j, int n
     const int first_row = (i == 0), last_row = (i == ny - 1);
     const int first_col = (j == 0), last_col = (j == nx - 1);
     if ((first_row || last_row) && (first_col || last_col)) {
-        /* corners, mask.cpp:86-89 */
+        /* corners, mask.cpp:73-76 */
         const float a = first_col ? v1[p] : -v1[p - 1];
         if (first_row) return a + v2[p];
         return a - v2[p - nx];
@@ -335,7 +335,7 @@ void orc_divergence(const float *v1, const float *v2, float *div, int nx, int ny
 }
 
 /* ------------------------------------------------------------------------- */
-/* normalisation (tvl1.cpp:344-371)                                          */
+/* normalisation (tvl1.cpp:302-329)                                          */
 /* ------------------------------------------------------------------------- */
 
 static size_t isamax(const float *x, size_t n)
@@ -378,10 +378,10 @@ void orc_image_normalization(const float *I0, const float *I1, float *I0n, float
 }
 
 /* ------------------------------------------------------------------------- */
-/* TV-L1 (tvl1.cpp:224-336)                                                  */
+/* TV-L1 (tvl1.cpp:182-294)                                                  */
 /* ------------------------------------------------------------------------- */
 
-/* Thresholding step TH for one pixel (tvl1.cpp:263-287). */
+/* Thresholding step TH for one pixel (tvl1.cpp:221-245). */
 static inline void th_step(float rho_c, float wx, float wy, float u1, float u2, float l_t,
                            float *v1, float *v2)
 {
@@ -413,7 +413,7 @@ void orc_tvl1_th(const float *rho_c, const float *wx, const float *wy, const flo
     for (size_t i = 0; i < n; i++) th_step(rho_c[i], wx[i], wy[i], u1[i], u2[i], l_t, v1 + i, v2 + i);
 }
 
-/* Warp set-up of one warping (tvl1.cpp:239-254): warped I1 and its warped
+/* Warp set-up of one warping (tvl1.cpp:198-212): warped I1 and its warped
  * derivatives; rho_c = I1w - I1wx*u1 - I1wy*u2 - I0 with u at warp start. */
 void orc_tvl1_warp_setup(const float *I0, const float *I1, const float *I1x, const float *I1y,
                          const float *u1, const float *u2, float *I1w, float *I1wx,
@@ -445,7 +445,7 @@ void orc_set_error_mode(int mode)
     g_error_mode = mode == ORC_ERROR_FLOAT_SEQUENTIAL ? mode : ORC_ERROR_DOUBLE_ROWS;
 }
 
-/* One primal-dual iteration from a given state (tvl1.cpp:260-333).  Updates
+/* One primal-dual iteration from a given state (tvl1.cpp:218-291).  Updates
  * u1,u2,p11..p22 in place and returns sum over pixels of
  * (du1^2 + du2^2) accumulated in double (each pixel term in float). */
 double orc_tvl1_iteration(const float *I1wx, const float *I1wy, const float *rho_c,
@@ -473,7 +473,7 @@ double orc_tvl1_iteration(const float *I1wx, const float *I1wy, const float *rho
     orc_divergence(p21, p22, div2, nx, ny);
 
     /* u update and the stopping error.  Each pixel term is the reference's
-     * float expression (tvl1.cpp:304-305); the sum runs in double, row by
+     * float expression (tvl1.cpp:262-263); the sum runs in double, row by
      * row, rows added in order, so it does not depend on the thread count. */
     #pragma omp parallel for
     for (int i = 0; i < ny; i++) {
@@ -504,7 +504,7 @@ double orc_tvl1_iteration(const float *I1wx, const float *I1wy, const float *rho
     }
     free(u1k_save);
 
-    /* forward gradients go into the div/v scratch as in tvl1.cpp:310-311 */
+    /* forward gradients go into the div/v scratch as in tvl1.cpp:268-269 */
     orc_forward_gradient(u1, div1, v1, nx, ny);
     orc_forward_gradient(u2, div2, v2, nx, ny);
     #pragma omp parallel for
@@ -524,7 +524,7 @@ double orc_tvl1_iteration(const float *I1wx, const float *I1wy, const float *rho
     return err;
 }
 
-/* One scale of dual TV-L1 (tvl1.cpp:224-336).  `iters` receives the
+/* One scale of dual TV-L1 (tvl1.cpp:182-294).  `iters` receives the
  * iteration count of each warping. */
 int orc_dual_tvl1(const float *I0, const float *I1, float *u1, float *u2, int nx, int ny,
                   const orc_tvl1_params *prm, int *iters)
@@ -538,7 +538,7 @@ int orc_dual_tvl1(const float *I0, const float *I1, float *u1, float *u2, int nx
     float *scratch = buf + 10 * size;
 
     orc_centered_gradient(I1, I1x, I1y, nx, ny);
-    /* p = 0 (calloc), tvl1.cpp:232-235 */
+    /* p = 0 (calloc), tvl1.cpp:190-193 */
     const float eps2 = prm->epsilon * prm->epsilon;
     for (int w = 0; w < prm->warps; w++) {
         orc_tvl1_warp_setup(I0, I1, I1x, I1y, u1, u2, I1w, I1wx, I1wy, rho_c, nx, ny);
@@ -594,8 +594,8 @@ void orc_tvl1_default_params(orc_tvl1_params *p)
     p->max_iterations = 300;
 }
 
-/* Multiscale driver, tvl1.cpp:171-217, from float frames already converted
- * (tvl1.cpp:160-165 converts u8 with a plain cast).  `u` receives planar
+/* Multiscale driver, tvl1.cpp:129-175, from float frames already converted
+ * (tvl1.cpp:118-123 converts u8 with a plain cast).  `u` receives planar
  * [u1 | u2] (2*W*H floats).  `iters` receives nscales*warps counts, finest
  * scale first.  Returns the number of scales used via *nscales_out. */
 /* The multiscale driver (tvl1.cpp:139-175).  The warm-start extension,
@@ -671,7 +671,7 @@ int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height
     return tvl1_run(I0in, I1in, width, height, prm, NULL, -1, u, iters, nscales_out);
 }
 
-/* tvl1.cpp:160-165 then the multiscale driver, from `uinit` (or zero). */
+/* tvl1.cpp:118-123 then the multiscale driver, from `uinit` (or zero). */
 int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
                          const orc_tvl1_params *prm, const float *uinit, int first, float *u,
                          int *iters, int *nscales_out)

@@ -387,7 +387,7 @@ int ls_begin(const of_tvl1_batch* b, LsLane& L, const std::vector<int>& members,
             E.wsrc = reinterpret_cast<float4*>(h->wsrc);
             E.p0 = h->pbase;
             E.dst = dU[members[i]];
-            if (s == h->ns - 1) E.ucoarse = ub;   // zero flow, tvl1.cpp:199-200
+            if (s == h->ns - 1) E.ucoarse = ub;   // zero flow, tvl1.cpp:157-158
             E.invz = 1 / P.zfactor;
             E.pair_id = (unsigned)i;
             unsigned gx, gy;
@@ -484,7 +484,7 @@ int ls_begin(const of_tvl1_batch* b, LsLane& L, const std::vector<int>& members,
     OF_HIP(hipMemcpyAsync(L.d_bases, L.h_bases, sizeof(unsigned) * q, hipMemcpyHostToDevice, main));
     OF_HIP(hipMemsetAsync(L.d_iters, 0, sizeof(int) * L.niters, main));
 
-    // phase 1 of all pairs (tvl1.cpp:181-196): frames to float with the
+    // phase 1 of all pairs (tvl1.cpp:137-154): frames to float with the
     // min/max keys, control blocks, normalisation, pre-smoothing, pyramids
     const GaussTaps& g_pre = hs[0]->g_pre;    // the same for every handle of the batch
     const GaussTaps& g_zoom = hs[0]->g_zoom;
@@ -497,7 +497,7 @@ int ls_begin(const of_tvl1_batch* b, LsLane& L, const std::vector<int>& members,
         OF_TRY(launch_lp_zoom_out(st_zoom[s], P.zfactor, main));
     }
 
-    // phase 2 starts at the coarsest scale (tvl1.cpp:201-216)
+    // phase 2 starts at the coarsest scale (tvl1.cpp:157-174)
     L.done = false;
     L.s = maxns - 1;
     L.w = 0;

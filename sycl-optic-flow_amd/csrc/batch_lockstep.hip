@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_ls_level_begin(LsPhase ph, const LsPair
         const int i = (int)(lb / P.bx) * 4 + row;
         if (j >= nx || i >= ny) return;
         gradient_packed_px(P.I1s, P.wsrc, nx, ny, i, j);
-        // p = 0 in buffer 0 (tvl1.cpp:232-235)
+        // p = 0 in buffer 0 (tvl1.cpp:190-193)
         const size_t N = (size_t)P.a.plane, p = (size_t)i * nx + j;
         P.p0[p] = 0.0f;
         P.p0[N + p] = 0.0f;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_ls_copy_flow(LsPhase ph, const LsPair* 
 }
 
 // ---------------------------------------------------------------------------
-// batched preparation (frames to pyramids, tvl1.cpp:160-196)
+// batched preparation (frames to pyramids, tvl1.cpp:137-154)
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_lp_load(LsPrepStep st, const LsPrep* __restrict__ preps)

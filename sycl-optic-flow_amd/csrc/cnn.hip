@@ -7,7 +7,7 @@
 // `hidden` x (conv3x3 C->C + ReLU), conv3x3 C->2; zero padding.
 //
 // Every convolution is an implicit GEMM on the bf16 MFMA
-// (v_mfma_f32_16x16x16_bf16, fp32 accumulation), with M = pixels:
+// (v_mfma_f32_16x16x32_bf16, fp32 accumulation), with M = pixels:
 //   forward        Y[m][n]  = sum_(t,c) X[m + s(t)][c] W[n][t][c]
 //   data gradient  dX[m][c] = sum_(t,n) dY[m - s(t)][n] W[n][t][c]
 //   weight grad.   dW[n][t][c] = sum_m dY[m][n] X[m + s(t)][c]

@@ -16,7 +16,7 @@ void record_hip_error(hipError_t e, const char* what, const char* file, int line
     (void)hipGetLastError();
 }
 
-// mask.cpp:225-244: size = (int)(5*sigma)+1 taps exp(-i^2/(2 sigma^2)) /
+// mask.cpp:212-231: size = (int)(5*sigma)+1 taps exp(-i^2/(2 sigma^2)) /
 // (sigma sqrt(2 pi)) in double, normalised by twice the |tap| sum minus the
 // centre tap (cblas_dasum, summed left to right), scaled by the reciprocal.
 int gaussian_taps(double sigma, GaussTaps* g)

@@ -64,7 +64,7 @@ void dev_free(T*& p)
     p = nullptr;
 }
 
-// Gaussian taps of mask.cpp:225-244 computed on the host in double with the
+// Gaussian taps of mask.cpp:212-231 computed on the host in double with the
 // C library's exp, so CPU and GPU blur with identical coefficients.
 struct GaussTaps {
     double B[64];

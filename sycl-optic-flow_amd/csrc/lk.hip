@@ -16,6 +16,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -507,7 +508,7 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
 
 // One pyramid step of both frames per launch (blockIdx.z = frame): the
 // Gaussian rows, the Gaussian columns, then zoom_out's resampling, with the
-// pixel bodies of the TV-L1 pyramid (mask.cpp:217-287, zoom.cpp:28-52).
+// pixel bodies of the TV-L1 pyramid (mask.cpp:204-274, zoom.cpp:28-52).
 __global__ __launch_bounds__(256) void k_lk_pyr_rows(const float* __restrict__ a,
                                                      const float* __restrict__ b,
                                                      float* __restrict__ oa, float* __restrict__ ob,
@@ -555,10 +556,15 @@ unsigned g1(size_t n) { const unsigned g = cdiv(n, 256); return g > 4096 ? 4096 
 
 }  // namespace
 
-// Device workspace of the pyramidal LK, one per host thread, grown on
-// demand and kept for the thread's lifetime (the entry points take no
-// handle).
+// Device workspace of the pyramidal LK (the entry points take no handle).
+// Workspaces live in a process-wide pool, one list per device: a call takes
+// one for its duration and gives it back idle (the call has synchronised its
+// stream, also on its error paths), so growing one needs no device-wide wait.
+// The pool is never destroyed: nothing in the library calls HIP while the
+// process exits (static and thread-local destructors run after the runtime
+// may have begun its own teardown); the driver reclaims the memory.
 struct LkWorkspace {
+    int device = -1;
     size_t cap_total = 0, cap_n0 = 0;
     float *P0 = nullptr, *P1 = nullptr, *Ix = nullptr, *Iy = nullptr;
     float *PROD = nullptr, *ROWS = nullptr, *GT = nullptr, *u = nullptr, *v = nullptr, *nu = nullptr,
@@ -568,10 +574,10 @@ struct LkWorkspace {
         for (float** p : {&P0, &P1, &Ix, &Iy, &PROD, &ROWS, &GT, &u, &v, &nu, &nv}) dev_free(*p);
         cap_total = cap_n0 = 0;
     }
+    // idle workspace: no kernel of an earlier call still uses the buffers
     int ensure(size_t total, size_t n0)
     {
         if (total <= cap_total && n0 <= cap_n0) return OF_OK;
-        if (hipDeviceSynchronize() != hipSuccess) return OF_ERR_HIP;
         release();
         int st = dev_alloc(&P0, total);
         if (!st) st = dev_alloc(&P1, total);
@@ -587,14 +593,60 @@ struct LkWorkspace {
         cap_n0 = n0;
         return OF_OK;
     }
-    ~LkWorkspace() { release(); }
 };
 
-LkWorkspace& lk_workspace()
+struct LkPool {
+    std::mutex m;
+    std::vector<LkWorkspace*> idle;
+};
+
+LkPool& lk_pool()
 {
-    thread_local LkWorkspace ws;
-    return ws;
+    static LkPool* pool = new LkPool();   // never deleted (see above)
+    return *pool;
 }
+
+// Takes an idle workspace of `device` for one call; gives it back when the
+// call ends, after draining `st` if the call returned early with work queued.
+class LkLease {
+public:
+    LkLease(int device, hipStream_t st) : st_(st)
+    {
+        LkPool& p = lk_pool();
+        std::lock_guard<std::mutex> g(p.m);
+        for (size_t i = 0; i < p.idle.size(); i++)
+            if (p.idle[i]->device == device) {
+                ws_ = p.idle[i];
+                p.idle.erase(p.idle.begin() + i);
+                return;
+            }
+        ws_ = new (std::nothrow) LkWorkspace();
+        if (ws_) ws_->device = device;
+    }
+    ~LkLease()
+    {
+        if (!ws_) return;
+        if (!drained_) {
+            (void)hipStreamSynchronize(st_);
+            (void)hipGetLastError();
+        }
+        LkPool& p = lk_pool();
+        std::lock_guard<std::mutex> g(p.m);
+        try {
+            p.idle.push_back(ws_);
+        } catch (...) {
+            ws_->release();
+            delete ws_;
+        }
+    }
+    LkWorkspace* get() const { return ws_; }
+    void drained() { drained_ = true; }
+
+private:
+    hipStream_t st_;
+    LkWorkspace* ws_ = nullptr;
+    bool drained_ = false;
+};
 
 // Pyramidal LK from device frames.
 int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, float zf,
@@ -618,7 +670,11 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
         total += (size_t)nx[l] * ny[l];
     }
     const size_t n0 = (size_t)W * H;
-    LkWorkspace& ws = lk_workspace();
+    int device = 0;
+    OF_HIP(hipGetDevice(&device));
+    LkLease lease(device, st);
+    if (!lease.get()) return OF_ERR_ALLOC;
+    LkWorkspace& ws = *lease.get();
     int status = ws.ensure(total, n0);
     float* P0 = ws.P0;
     float* P1 = ws.P1;
@@ -731,6 +787,7 @@ int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, f
             OF_HIP(hipMemcpyAsync(dv, v, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
         }
         OF_HIP(hipStreamSynchronize(st));
+        lease.drained();
         return OF_OK;
     };
     if (status == OF_OK) status = run();

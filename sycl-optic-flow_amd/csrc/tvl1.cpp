@@ -275,10 +275,10 @@ int amdflow::tvl1_prepare(of_tvl1* h, const void* src0, const void* src1, bool i
     float* I00 = h->I0p;
     float* I10 = h->I1p;
     OF_TRY(launch_normalize(h->A0, h->A1, I00, I10, N0, h->ctl, st));
-    // pre-smoothing, tvl1.cpp:185-186
+    // pre-smoothing, tvl1.cpp:143-144
     OF_TRY(launch_gaussian(I00, h->A0, I00, h->W, h->H, h->g_pre, st));
     OF_TRY(launch_gaussian(I10, h->A0, I10, h->W, h->H, h->g_pre, st));
-    // pyramid, tvl1.cpp:189-196 (zoom_out: blur a copy, then resample)
+    // pyramid, tvl1.cpp:146-154 (zoom_out: blur a copy, then resample)
     for (int s = 1; s < h->ns; s++) {
         for (int k = 0; k < 2; k++) {
             float* P0 = (k ? h->I1p : h->I0p);
@@ -297,7 +297,7 @@ namespace {
 
 // Second phase: the coarse-to-fine solve from the prepared pyramids, with
 // `init` (planar [u1|u2] at the coarsest scale, device) as the initial flow,
-// or zero when null (tvl1.cpp:199-200); at most `warps` warpings per scale
+// or zero when null (tvl1.cpp:157-158); at most `warps` warpings per scale
 // and `max_iter` iterations per warping.  Waits for the result.
 int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_iter, float* dst,
                  hipStream_t st)
@@ -324,7 +324,7 @@ int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_it
         float* ub = h->ubase + 4 * h->off[s];
         if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s], st));
         OF_TRY(launch_centered_gradient_packed(I1s, h->wsrc, nx, ny, st));
-        OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:232-235
+        OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:190-193
         for (int w = 0; w < warps; w++) {
             OF_TRY(launch_warp_setup(I0s, h->wsrc, ub, ub + 2 * N, h->wx, h->wy, h->rhoc, nx, ny,
                                      h->ctl, s, st));
@@ -485,7 +485,7 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         else of_tvl1_default_params(&p);
         OF_TRY(validate(p, width, height));
     
-        // scale count, tvl1.cpp:84-85, clamped to >= 1
+        // scale count, tvl1.cpp:42-43, clamped to >= 1
         const float logz = std::log(1.0f / p.zfactor);
         const double q = std::log(std::hypot((double)width, (double)height) / 16.0) / (double)logz;
         const float Nf = (float)(1.0 + q);

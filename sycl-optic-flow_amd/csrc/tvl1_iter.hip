@@ -1,4 +1,4 @@
-// tvl1_iter.hip -- the TV-L1 primal-dual iterations (tvl1.cpp:256-334).
+// tvl1_iter.hip -- the TV-L1 primal-dual iterations (tvl1.cpp:214-292).
 //
 // Two drivers share one tile core:
 //   k_tvl1_iter<K,R>     one launch = K iterations; the last workgroup to
@@ -19,7 +19,7 @@
 namespace amdflow {
 
 // Divergence at one pixel from the current value and the left/up neighbours
-// with the border rules of mask.cpp:44-90.  Written with selects; every
+// with the border rules of mask.cpp:31-77.  Written with selects; every
 // candidate is the exact expression of its reference case (the first/last
 // column cases associate left to right as written at :81-82).
 __device__ __forceinline__ float div_rule(float v1, float v1l, float v2, float v2u, bool fr,
@@ -33,7 +33,7 @@ __device__ __forceinline__ float div_rule(float v1, float v1l, float v2, float v
     return fr ? t_first_row : (lr ? t_last_row : ((fc || lc) ? t_edge_col : t_body));
 }
 
-// Thresholding operator TH, tvl1.cpp:263-287, branch-free: the three
+// Thresholding operator TH, tvl1.cpp:221-245, branch-free: the three
 // reference cases multiply the warped gradient by l_t, -l_t or -rho/grad,
 // and the `grad < 1e-10` case stores +0 exactly as the reference does.
 __device__ __forceinline__ void th_step(float rc, float gx, float gy, float u1, float u2,
@@ -197,7 +197,7 @@ __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int
                 const float fy2 = yl ? u2d - u2[r] : 0.0f;
                 const float g1 = 1.0f + taut * hypot_fin(fx1, fy1);
                 const float g2 = 1.0f + taut * hypot_fin(fx2, fy2);
-                // cblas_saxpy (p += taut*grad) then the division, tvl1.cpp:322-333
+                // cblas_saxpy (p += taut*grad) then the division, tvl1.cpp:280-291
                 p11[r] = (p11[r] + taut * fx1) / g1;
                 p12[r] = (p12[r] + taut * fy1) / g1;
                 p21[r] = (p21[r] + taut * fx2) / g2;
@@ -327,7 +327,7 @@ __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int b
             const float fy2 = yl ? u2d - u2 : 0.0f;
             const float g1 = 1.0f + taut * hypot_fin(fx1, fy1);
             const float g2 = 1.0f + taut * hypot_fin(fx2, fy2);
-            // cblas_saxpy (p += taut*grad) then the division, tvl1.cpp:322-333
+            // cblas_saxpy (p += taut*grad) then the division, tvl1.cpp:280-291
             p11 = (p11 + taut * fx1) / g1;
             p12 = (p12 + taut * fy1) / g1;
             p21 = (p21 + taut * fx2) / g2;
@@ -408,7 +408,7 @@ __device__ __forceinline__ void grid_sums(const double* part, unsigned nb, int k
 // The stopping rule applied to kk error sums after iteration n0 (1-based
 // iterations n0+1 .. n0+kk): returns the number of iterations to keep when
 // the warping stops inside or at the end of this chunk, 0 to continue.
-// tvl1.cpp:256-258: continue while error > epsilon^2 and n < max.
+// tvl1.cpp:214-216: continue while error > epsilon^2 and n < max.
 __device__ __forceinline__ int stop_after(const double* tot, int kk, int n0, const IterArgs& a)
 {
     for (int t = 0; t < kk; t++) {

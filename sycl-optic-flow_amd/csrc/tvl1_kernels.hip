@@ -46,7 +46,7 @@ __global__ void k_ctl_begin(TvCtl* ctl)
 }
 
 // ---------------------------------------------------------------------------
-// input conversion + joint min/max (tvl1.cpp:160-165, 352-361)
+// input conversion + joint min/max (tvl1.cpp:118-123, 310-319)
 // ---------------------------------------------------------------------------
 
 // Converts both frames to float and finds, per frame, the first index of the
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_minmax_final(TvCtl* ctl, unsigned nbloc
     }
 }
 
-// image_normalization, tvl1.cpp:344-371: max = std::max of the two isamax
+// image_normalization, tvl1.cpp:302-329: max = std::max of the two isamax
 // picks, min = std::min of the two isamin picks, den = max - min; when
 // den <= 0 the frames are copied unchanged, else 255*(I-min)/den in double.
 __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ a,
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ a,
 }
 
 // ---------------------------------------------------------------------------
-// separable Gaussian (mask.cpp:217-287)
+// separable Gaussian (mask.cpp:204-274)
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_gauss_rows(const float* __restrict__ src,
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_zoom_out_sample(const float* __restrict
 }
 
 // zoom_in of both flow components (zoom.cpp:61-85) fused with the
-// cblas_sscal by 1/zfactor (tvl1.cpp:213-214).  Reads the coarse scale's
+// cblas_sscal by 1/zfactor (tvl1.cpp:171-172).  Reads the coarse scale's
 // current buffer, writes buffer 0 of the finer scale.
 __global__ __launch_bounds__(256) void k_zoom_in_flow(const float* __restrict__ uc0,
                                                       const float* __restrict__ uc1,
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_zoom_in_planes(const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// centred gradient (mask.cpp:152-209)
+// centred gradient (mask.cpp:139-196)
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_centered_gradient(const float* __restrict__ in,
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, unsigned n,
 }
 
 // ---------------------------------------------------------------------------
-// warping set-up (tvl1.cpp:237-254)
+// warping set-up (tvl1.cpp:198-212)
 // ---------------------------------------------------------------------------
 
 // Warps I1, I1x, I1y (interleaved, one float4 per pixel) by the current flow

@@ -11,7 +11,7 @@
 
 namespace amdflow {
 
-// (I1, I1x, I1y, 0) at pixel (i, j): the centred gradient of mask.cpp:152-209
+// (I1, I1x, I1y, 0) at pixel (i, j): the centred gradient of mask.cpp:139-196
 // (one-sided differences times 0.5 on the borders, evaluated in double).
 __device__ __forceinline__ void gradient_packed_px(const float* __restrict__ in,
                                                    float4* __restrict__ out, int nx, int ny, int i,
@@ -25,7 +25,7 @@ __device__ __forceinline__ void gradient_packed_px(const float* __restrict__ in,
                                           (float)(0.5 * (double)ddy), 0.0f);
 }
 
-// Warping set-up at pixel (i, j) (tvl1.cpp:237-254): I1, I1x, I1y sampled
+// Warping set-up at pixel (i, j) (tvl1.cpp:198-212): I1, I1x, I1y sampled
 // at x + u with border_out = true, then I1wx, I1wy and
 // rho_c = I1w - I1wx*u1 - I1wy*u2 - I0.  `ub` is the current flow buffer
 // (u1 plane, u2 plane at +nx*ny).
@@ -49,7 +49,7 @@ __device__ __forceinline__ void warp_setup_px(const float* __restrict__ I0,
 }
 
 // zoom_in of both flow components at fine pixel (i1, j1) (zoom.cpp:61-85)
-// times 1/zfactor (tvl1.cpp:213-214).  `uc` is the coarse scale's current
+// times 1/zfactor (tvl1.cpp:171-172).  `uc` is the coarse scale's current
 // buffer, `uf` the fine scale's buffer 0.
 __device__ __forceinline__ void zoom_in_flow_px(const float* __restrict__ uc,
                                                 float* __restrict__ uf, int nx, int ny, int nxx,
@@ -70,7 +70,7 @@ __device__ __forceinline__ void zoom_in_flow_px(const float* __restrict__ uc,
 // phase 1: normalisation, Gaussian, pyramid sampling
 // ---------------------------------------------------------------------------
 
-// image_normalization bounds (tvl1.cpp:344-371) from the min/max keys of
+// image_normalization bounds (tvl1.cpp:302-329) from the min/max keys of
 // both frames: mx = max of the two isamax picks, mn = min of the two isamin
 // picks, den = mx - mn.
 __device__ __forceinline__ void norm_bounds(const TvCtl* ctl, const float* a, const float* b,
@@ -130,11 +130,11 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 }
 
 // Low side: whole-sample mirror x[-k] = x[k]; high side: half-sample mirror
-// x[n-1+k] = x[n-k] (mask.cpp:253-255, 274-276).
+// x[n-1+k] = x[n-k] (mask.cpp:240-242, 261-263).
 __device__ __forceinline__ int mirror_lo(int i) { return i < 0 ? -i : i; }
 __device__ __forceinline__ int mirror_hi(int i, int n) { return i >= n ? 2 * n - 1 - i : i; }
 
-// Separable Gaussian (mask.cpp:217-287), double accumulation: the row pass
+// Separable Gaussian (mask.cpp:204-274), double accumulation: the row pass
 // and the column pass at pixel (y, x).
 __device__ __forceinline__ float gauss_row_px(const float* __restrict__ src, int nx, int x, int y,
                                               const GaussTaps& g)

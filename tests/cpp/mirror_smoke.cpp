@@ -42,11 +42,21 @@ int main()
         std::printf("TV-L1 mean flow (%.3f, %.3f), expected (%.2f, %.2f)\n", su / n, sv / n, du, dv);
         if (std::fabs(su / n - du) > 0.05 || std::fabs(sv / n - dv) > 0.05) return 1;
 
-        of_hs* hs = amdflow::initFlow(1, W, W, H);
+        // the reference's global-state calls, src/hs/cuda/main.cu:154,180
+        amdflow::initFlow(1, W, W, H);
         std::vector<float> hu(W * H), hv(W * H);
-        amdflow::ComputeFlow(hs, F0.data(), F1.data(), W, H, W, 15.0f, 1, 3, 200, hu.data(), hv.data());
-        amdflow::deleteFlow_mem(hs);
+        amdflow::ComputeFlow(F0.data(), F1.data(), W, H, W, 15.0f, 1, 3, 200, hu.data(), hv.data());
+        amdflow::deleteFlow_mem(1);
         std::printf("HS centre flow (%.3f, %.3f)\n", hu[H / 2 * W + W / 2], hv[H / 2 * W + W / 2]);
+        // the handle form gives the same flow
+        of_hs* hs = amdflow::initFlowHandle(1, W, W, H);
+        std::vector<float> hu2(W * H), hv2(W * H);
+        amdflow::ComputeFlow(hs, F0.data(), F1.data(), W, H, W, 15.0f, 1, 3, 200, hu2.data(), hv2.data());
+        amdflow::deleteFlow_mem(hs);
+        if (hu != hu2 || hv != hv2) {
+            std::printf("HS handle form differs from the global-state form\n");
+            return 1;
+        }
 
         amdflow::LucasKanade lk(3, 2, 5, W, H);
         lk.copy_frames_circularbuffer_GPU_wrapper(I0.data(), 2, 0, W * H);

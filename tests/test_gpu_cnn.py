@@ -382,7 +382,7 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained warm-start weights not present")
 def test_trained_network_beats_zero_init_at_same_limits():
-    """At 1920x1080, starting at 960x540 with at most 2 warpings and 100
+    """At 1920x1080, starting at 480x270 with at most 2 warpings and 100
     iterations: from the trained network's prediction the solve stays close
     to the plain solve and to the true motion; from zero flow it does not."""
     from amdflow.cnn import WarmStartNet

@@ -282,7 +282,7 @@ def test_normalization_properties():
 
 def test_normalization_uses_abs_extremes_for_signed_input():
     # isamax/isamin pick the largest/smallest |x|: with negative values the
-    # "max" is the value of largest magnitude (cblas semantics, tvl1.cpp:353-356)
+    # "max" is the value of largest magnitude (cblas semantics, tvl1.cpp:311-314)
     a = np.array([[-10.0, 2.0, 3.0]], np.float32)
     b = np.array([[1.0, 4.0, 5.0]], np.float32)
     na, nb = O.image_normalization(a, b)
