@@ -30,7 +30,9 @@ typedef enum {
     OF_ERR_ALLOC = 3,        /* device or host allocation failed */
     OF_ERR_HIP = 4,          /* a HIP runtime call failed */
     OF_ERR_STATE = 5,        /* call not valid in the handle's state */
-    OF_ERR_UNSUPPORTED = 6   /* filter size or option not provided */
+    OF_ERR_UNSUPPORTED = 6,  /* filter size or option not provided */
+    OF_ERR_CAPACITY = 7      /* input larger than the fixed scratch of an
+                                entry point that does not allocate */
 } of_status;
 
 const char *of_status_string(int status);
@@ -406,6 +408,22 @@ int of_lk_pyr_compute(const float *I0, const float *I1, int width, int height, i
 int of_lk_pyr_compute_device(const float *dI0, const float *dI1, int width, int height,
                              int levels, float zf, float sigma, int window, int iters,
                              float det_eps, float *du, float *dv, void *stream);
+
+/* No reference counterpart.  The same solve as of_lk_pyr_compute_device,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it calls
+ * no hipMalloc/hipFree or allocating routine and keeps its workspace in a
+ * fixed 128 MiB block of device memory per GPU, held from the library's
+ * first use on that GPU until the process ends.  The workspace needs
+ * 4 bytes x (2 x (pixels of levels 1..) + 15 x width*height), each part
+ * rounded up to 256 bytes: up to 1920x1080 at any level count.  Larger
+ * inputs return OF_ERR_CAPACITY.  Every call on one GPU shares that block:
+ * calls that overlap in time on one GPU (other host threads, other streams)
+ * race and must be serialised by the caller.  Results are bit-identical to
+ * of_lk_pyr_compute_device. */
+int of_lk_pyr_compute_device_noalloc(const float *dI0, const float *dI1, int width, int height,
+                                     int levels, float zf, float sigma, int window, int iters,
+                                     float det_eps, float *du, float *dv, void *stream);
 
 #ifdef __cplusplus
 }

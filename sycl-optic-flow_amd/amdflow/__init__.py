@@ -135,6 +135,9 @@ SIGNATURES = {
                                     C.c_int, C.c_int, C.c_float, _vp, _vp]),
     "of_lk_pyr_compute_device": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
                                            C.c_float, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp]),
+    "of_lk_pyr_compute_device_noalloc": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                                   C.c_float, C.c_int, C.c_int, C.c_float, _vp,
+                                                   _vp, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

@@ -48,6 +48,7 @@ extern "C" const char* of_status_string(int status)
     case OF_ERR_HIP: return amdflow::g_last_error[0] ? amdflow::g_last_error : "HIP runtime error";
     case OF_ERR_STATE: return "call not valid in this state";
     case OF_ERR_UNSUPPORTED: return "unsupported option";
+    case OF_ERR_CAPACITY: return "input larger than the fixed scratch of a no-allocation entry point";
     default: return "unknown status";
     }
 }

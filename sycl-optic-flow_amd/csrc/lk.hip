@@ -563,34 +563,228 @@ unsigned g1(size_t n) { const unsigned g = cdiv(n, 256); return g > 4096 ? 4096 
 // The pool is never destroyed: nothing in the library calls HIP while the
 // process exits (static and thread-local destructors run after the runtime
 // may have begun its own teardown); the driver reclaims the memory.
+// Geometry of one pyramidal LK solve: level sizes, and the offsets of
+// levels 1.. inside the pyramid planes P0/P1 (level 0 is read from the
+// caller's frames in place).
+struct LkPlan {
+    int levels = 0;
+    int nx[16], ny[16];
+    size_t off[16];
+    size_t pyr = 0;   // floats of levels 1.. per frame
+    size_t n0 = 0;    // pixels of level 0
+    GaussTaps g;
+};
+
+int lk_plan(int W, int H, int levels, float zf, float sigma, int win, int iters, LkPlan* P)
+{
+    if (W < 2 || H < 2) return OF_ERR_ARG;
+    if (levels < 1 || levels > 16 || win < 1 || (win & 1) == 0 || iters < 0) return OF_ERR_ARG;
+    if (!(zf > 0 && zf < 1) || !(sigma > 0)) return OF_ERR_ARG;
+    OF_TRY(gaussian_taps((double)sigma, &P->g));
+    P->levels = levels;
+    P->nx[0] = W;
+    P->ny[0] = H;
+    P->off[0] = 0;
+    P->pyr = 0;
+    for (int l = 0; l < levels; l++) {
+        if (l) zoom_size(P->nx[l - 1], P->ny[l - 1], zf, &P->nx[l], &P->ny[l]);
+        if (P->nx[l] < 2 || P->ny[l] < 2) return OF_ERR_ARG;
+        if (l + 1 < levels && (P->g.size > P->nx[l] || P->g.size > P->ny[l])) return OF_ERR_SIGMA;
+        if (l) {
+            P->off[l] = P->pyr;
+            P->pyr += (size_t)P->nx[l] * P->ny[l];
+        }
+    }
+    P->n0 = (size_t)W * H;
+    return OF_OK;
+}
+
+// Device buffers of one solve, carved from one block of lk_block_floats().
+struct LkBufs {
+    float *P0, *P1;            // pyramid levels 1.. of each frame (LkPlan::off)
+    float *Ix, *Iy, *u, *v, *nu, *nv;   // one plane of level 0's size each
+    float *PROD, *ROWS, *GT;   // three planes each: per-pixel products, their
+                               // row sums, the structure tensor Gxx, Gxy, Gyy
+};
+
+size_t lk_round(size_t n) { return (n + 63) & ~(size_t)63; }   // 256-byte multiples
+
+size_t lk_block_floats(const LkPlan& P)
+{
+    return 2 * lk_round(P.pyr) + 6 * lk_round(P.n0) + 3 * lk_round(3 * P.n0);
+}
+
+LkBufs lk_carve(float* base, const LkPlan& P)
+{
+    LkBufs B;
+    float* q = base;
+    auto take = [&](size_t n) { float* r = q; q += lk_round(n); return r; };
+    B.P0 = take(P.pyr);
+    B.P1 = take(P.pyr);
+    B.Ix = take(P.n0);
+    B.Iy = take(P.n0);
+    B.u = take(P.n0);
+    B.v = take(P.n0);
+    B.nu = take(P.n0);
+    B.nv = take(P.n0);
+    B.PROD = take(3 * P.n0);
+    B.ROWS = take(3 * P.n0);
+    B.GT = take(3 * P.n0);
+    return B;
+}
+
+// The solve proper: pyramid, then coarse to fine the structure tensor and
+// `iters` iterations per level, zoom_in between levels.  Every buffer of B
+// is written before it is read (u, v are cleared at the coarsest level), so
+// a block that held an earlier call's data gives the same result.  Returns
+// after synchronising `st`.
+int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* dI1, float zf,
+               int win, int iters, float det_eps, float* du, float* dv, hipStream_t st)
+{
+    const int levels = P.levels;
+    const int* nx = P.nx;
+    const int* ny = P.ny;
+    const size_t n0 = P.n0;
+    const GaussTaps& g = P.g;
+    float* P0 = B.P0;
+    float* P1 = B.P1;
+    float* Ix = B.Ix;
+    float* Iy = B.Iy;
+    float* PROD = B.PROD;
+    float* ROWS = B.ROWS;
+    float* GT = B.GT;
+    float* u = B.u;
+    float* v = B.v;
+    float* nu = B.nu;
+    float* nv = B.nv;
+    size_t off[16];
+    for (int l = 0; l < levels; l++) off[l] = P.off[l];
+    // level 0 is read from the caller's frames; levels 1.. live in P0/P1.
+    // Each pyramid step treats both frames in one launch, with PROD and
+    // ROWS (free until the structure tensor) as the two blur buffers.
+    const float* L0[16];
+    const float* L1[16];
+    for (int l = 0; l < levels; l++) {
+        L0[l] = l ? P0 + off[l] : dI0;
+        L1[l] = l ? P1 + off[l] : dI1;
+    }
+    for (int l = 1; l < levels; l++) {
+        const int pw = nx[l - 1], ph = ny[l - 1];
+        const dim3 gb(cdiv(pw, 64), cdiv(ph, 4), 2), gz(cdiv(nx[l], 64), cdiv(ny[l], 4), 2);
+        hipLaunchKernelGGL(k_lk_pyr_rows, gb, dim3(256), 0, st, L0[l - 1], L1[l - 1], PROD,
+                           PROD + n0, pw, ph, g);
+        LK_CHECK("k_lk_pyr_rows");
+        hipLaunchKernelGGL(k_lk_pyr_cols, gb, dim3(256), 0, st, PROD, PROD + n0, ROWS, ROWS + n0,
+                           pw, ph, g);
+        LK_CHECK("k_lk_pyr_cols");
+        hipLaunchKernelGGL(k_lk_pyr_zoom, gz, dim3(256), 0, st, ROWS, ROWS + n0, P0 + off[l],
+                           P1 + off[l], pw, ph, nx[l], ny[l], zf);
+        LK_CHECK("k_lk_pyr_zoom");
+    }
+    const int wc = (win - 1) / 2;
+    // the last fused iteration at level 0 writes the caller's du/dv
+    // directly unless they overlap the frames or each other
+    // address ranges of len floats, compared as integers (the buffers
+    // are separate allocations)
+    auto overlaps = [](const float* a, const float* b, size_t len) {
+        const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b, n = len * sizeof(float);
+        return x < y + n && y < x + n;
+    };
+    const bool direct = wc >= 1 && wc <= 4 && iters >= 1 && !overlaps(du, dv, n0) &&
+                        !overlaps(du, dI0, n0) && !overlaps(du, dI1, n0) &&
+                        !overlaps(dv, dI0, n0) && !overlaps(dv, dI1, n0);
+    const float invZ = 1 / zf;
+    {
+        const size_t nc = (size_t)nx[levels - 1] * ny[levels - 1];
+        hipLaunchKernelGGL(k_lk_zero2, dim3(g1(nc)), dim3(256), 0, st, u, v, nc);
+        LK_CHECK("k_lk_zero2");
+    }
+    for (int l = levels - 1; l >= 0; l--) {
+        const int w = nx[l], h = ny[l];
+        const float* p0 = L0[l];
+        const float* p1 = L1[l];
+        const dim3 fgrid(cdiv(w, 64), cdiv(h, kLkTileRows));
+        if (wc >= 1 && wc <= 4) {
+            // gradient, products and both window sums in one launch
+            if (wc == 1) hipLaunchKernelGGL((k_lk_tensor_fused<1>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+            if (wc == 2) hipLaunchKernelGGL((k_lk_tensor_fused<2>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+            if (wc == 3) hipLaunchKernelGGL((k_lk_tensor_fused<3>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+            if (wc == 4) hipLaunchKernelGGL((k_lk_tensor_fused<4>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
+            LK_CHECK("k_lk_tensor_fused");
+        } else {
+            hipLaunchKernelGGL(k_lk_tensor_products, grid2d(w, h), dim3(256), 0, st, p0, w, h, Ix,
+                               Iy, PROD);
+            LK_CHECK("k_lk_tensor_products");
+            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 3, wc,
+                               0);
+            LK_CHECK("k_lk_boxsum");
+            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, 3, wc,
+                               1);
+            LK_CHECK("k_lk_boxsum");
+        }
+        for (int it = 0; it < iters && wc >= 1 && wc <= 4; it++) {
+            // the whole iteration in one kernel, (u, v) -> (nu, nv)
+            if (direct && l == 0 && it == iters - 1) {
+                nu = du;
+                nv = dv;
+            }
+            if (wc == 1) hipLaunchKernelGGL((k_lk_iter_fused<1>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+            if (wc == 2) hipLaunchKernelGGL((k_lk_iter_fused<2>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+            if (wc == 3) hipLaunchKernelGGL((k_lk_iter_fused<3>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+            if (wc == 4) hipLaunchKernelGGL((k_lk_iter_fused<4>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
+            LK_CHECK("k_lk_iter_fused");
+            float* t = u; u = nu; nu = t;
+            t = v; v = nv; nv = t;
+        }
+        for (int it = 0; it < iters && (wc < 1 || wc > 4); it++) {
+            hipLaunchKernelGGL(k_lk_it_products, grid2d(w, h), dim3(256), 0, st, p0, p1, Ix, Iy,
+                               u, v, w, h, PROD);
+            LK_CHECK("k_lk_it_products");
+            hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 2, wc, 0);
+            LK_CHECK("k_lk_boxsum");
+            hipLaunchKernelGGL(k_lk_update, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, wc,
+                               det_eps, u, v);
+            LK_CHECK("k_lk_update");
+        }
+        if (l > 0) {
+            // zoom_in (zoom.cpp:61-85) of both components, times 1/zf
+            OF_TRY(launch_zoom_in_planes(u, v, nu, nv, w, h, nx[l - 1], ny[l - 1], invZ, st));
+            float* t = u; u = nu; nu = t;
+            t = v; v = nv; nv = t;
+        }
+    }
+    if (!direct) {
+        OF_HIP(hipMemcpyAsync(du, u, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        OF_HIP(hipMemcpyAsync(dv, v, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    OF_HIP(hipStreamSynchronize(st));
+    return OF_OK;
+}
+
+// Device workspace of the pyramidal LK (the entry points take no handle):
+// one block of lk_block_floats().  Workspaces live in a process-wide pool,
+// one list per device: a call takes one for its duration and gives it back
+// idle (the call has synchronised its stream, also on its error paths), so
+// growing one needs no device-wide wait.  The pool is never destroyed:
+// nothing in the library calls HIP while the process exits (static and
+// thread-local destructors run after the runtime may have begun its own
+// teardown); the driver reclaims the memory.
 struct LkWorkspace {
     int device = -1;
-    size_t cap_total = 0, cap_n0 = 0;
-    float *P0 = nullptr, *P1 = nullptr, *Ix = nullptr, *Iy = nullptr;
-    float *PROD = nullptr, *ROWS = nullptr, *GT = nullptr, *u = nullptr, *v = nullptr, *nu = nullptr,
-          *nv = nullptr;
+    size_t cap = 0;
+    float* block = nullptr;
     void release()
     {
-        for (float** p : {&P0, &P1, &Ix, &Iy, &PROD, &ROWS, &GT, &u, &v, &nu, &nv}) dev_free(*p);
-        cap_total = cap_n0 = 0;
+        dev_free(block);
+        cap = 0;
     }
-    // idle workspace: no kernel of an earlier call still uses the buffers
-    int ensure(size_t total, size_t n0)
+    // idle workspace: no kernel of an earlier call still uses the block
+    int ensure(size_t floats)
     {
-        if (total <= cap_total && n0 <= cap_n0) return OF_OK;
+        if (floats <= cap) return OF_OK;
         release();
-        int st = dev_alloc(&P0, total);
-        if (!st) st = dev_alloc(&P1, total);
-        for (float** p : {&Ix, &Iy, &u, &v, &nu, &nv})
-            if (!st) st = dev_alloc(p, n0);
-        for (float** p : {&PROD, &ROWS, &GT})
-            if (!st) st = dev_alloc(p, 3 * n0);
-        if (st) {
-            release();
-            return st;
-        }
-        cap_total = total;
-        cap_n0 = n0;
+        OF_TRY(dev_alloc(&block, floats));
+        cap = floats;
         return OF_OK;
     }
 };
@@ -648,153 +842,56 @@ private:
     bool drained_ = false;
 };
 
-// Pyramidal LK from device frames.
+// Pyramidal LK from device frames, on a pooled workspace.
 int lk_pyr_solve(const float* dI0, const float* dI1, int W, int H, int levels, float zf,
                  float sigma, int win, int iters, float det_eps, float* du, float* dv,
                  hipStream_t st)
 {
-    if (!dI0 || !dI1 || !du || !dv || W < 2 || H < 2) return OF_ERR_ARG;
-    if (levels < 1 || levels > 16 || win < 1 || (win & 1) == 0 || iters < 0) return OF_ERR_ARG;
-    if (!(zf > 0 && zf < 1) || !(sigma > 0)) return OF_ERR_ARG;
-    GaussTaps g;
-    OF_TRY(gaussian_taps((double)sigma, &g));
-    int nx[16], ny[16];
-    size_t off[16];
-    nx[0] = W; ny[0] = H;
-    size_t total = 0;
-    for (int l = 0; l < levels; l++) {
-        if (l) zoom_size(nx[l - 1], ny[l - 1], zf, &nx[l], &ny[l]);
-        if (nx[l] < 2 || ny[l] < 2) return OF_ERR_ARG;
-        if (l + 1 < levels && (g.size > nx[l] || g.size > ny[l])) return OF_ERR_SIGMA;
-        off[l] = total;
-        total += (size_t)nx[l] * ny[l];
-    }
-    const size_t n0 = (size_t)W * H;
+    if (!dI0 || !dI1 || !du || !dv) return OF_ERR_ARG;
+    LkPlan P;
+    OF_TRY(lk_plan(W, H, levels, zf, sigma, win, iters, &P));
     int device = 0;
     OF_HIP(hipGetDevice(&device));
     LkLease lease(device, st);
     if (!lease.get()) return OF_ERR_ALLOC;
-    LkWorkspace& ws = *lease.get();
-    int status = ws.ensure(total, n0);
-    float* P0 = ws.P0;
-    float* P1 = ws.P1;
-    float* Ix = ws.Ix;
-    float* Iy = ws.Iy;
-    float* PROD = ws.PROD;   // per-pixel products (3 or 2 planes of the level size)
-    float* ROWS = ws.ROWS;   // their row sums
-    float* GT = ws.GT;       // structure tensor Gxx, Gxy, Gyy of the level
-    float* u = ws.u;
-    float* v = ws.v;
-    float* nu = ws.nu;
-    float* nv = ws.nv;
-    auto run = [&]() -> int {
-        // level 0 is read from the caller's frames; levels 1.. live in P0/P1.
-        // Each pyramid step treats both frames in one launch, with PROD and
-        // ROWS (free until the structure tensor) as the two blur buffers.
-        const float* L0[16];
-        const float* L1[16];
-        for (int l = 0; l < levels; l++) {
-            L0[l] = l ? P0 + off[l] : dI0;
-            L1[l] = l ? P1 + off[l] : dI1;
-        }
-        for (int l = 1; l < levels; l++) {
-            const int pw = nx[l - 1], ph = ny[l - 1];
-            const dim3 gb(cdiv(pw, 64), cdiv(ph, 4), 2), gz(cdiv(nx[l], 64), cdiv(ny[l], 4), 2);
-            hipLaunchKernelGGL(k_lk_pyr_rows, gb, dim3(256), 0, st, L0[l - 1], L1[l - 1], PROD,
-                               PROD + n0, pw, ph, g);
-            LK_CHECK("k_lk_pyr_rows");
-            hipLaunchKernelGGL(k_lk_pyr_cols, gb, dim3(256), 0, st, PROD, PROD + n0, ROWS, ROWS + n0,
-                               pw, ph, g);
-            LK_CHECK("k_lk_pyr_cols");
-            hipLaunchKernelGGL(k_lk_pyr_zoom, gz, dim3(256), 0, st, ROWS, ROWS + n0, P0 + off[l],
-                               P1 + off[l], pw, ph, nx[l], ny[l], zf);
-            LK_CHECK("k_lk_pyr_zoom");
-        }
-        const int wc = (win - 1) / 2;
-        // the last fused iteration at level 0 writes the caller's du/dv
-        // directly unless they overlap the frames or each other
-        // address ranges of len floats, compared as integers (the buffers
-        // are separate allocations)
-        auto overlaps = [](const float* a, const float* b, size_t len) {
-            const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b, n = len * sizeof(float);
-            return x < y + n && y < x + n;
-        };
-        const bool direct = wc >= 1 && wc <= 4 && iters >= 1 && !overlaps(du, dv, n0) &&
-                            !overlaps(du, dI0, n0) && !overlaps(du, dI1, n0) &&
-                            !overlaps(dv, dI0, n0) && !overlaps(dv, dI1, n0);
-        const float invZ = 1 / zf;
-        {
-            const size_t nc = (size_t)nx[levels - 1] * ny[levels - 1];
-            hipLaunchKernelGGL(k_lk_zero2, dim3(g1(nc)), dim3(256), 0, st, u, v, nc);
-            LK_CHECK("k_lk_zero2");
-        }
-        for (int l = levels - 1; l >= 0; l--) {
-            const int w = nx[l], h = ny[l];
-            const float* p0 = L0[l];
-            const float* p1 = L1[l];
-            const dim3 fgrid(cdiv(w, 64), cdiv(h, kLkTileRows));
-            if (wc >= 1 && wc <= 4) {
-                // gradient, products and both window sums in one launch
-                if (wc == 1) hipLaunchKernelGGL((k_lk_tensor_fused<1>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
-                if (wc == 2) hipLaunchKernelGGL((k_lk_tensor_fused<2>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
-                if (wc == 3) hipLaunchKernelGGL((k_lk_tensor_fused<3>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
-                if (wc == 4) hipLaunchKernelGGL((k_lk_tensor_fused<4>), fgrid, dim3(256), 0, st, p0, w, h, Ix, Iy, GT);
-                LK_CHECK("k_lk_tensor_fused");
-            } else {
-                hipLaunchKernelGGL(k_lk_tensor_products, grid2d(w, h), dim3(256), 0, st, p0, w, h, Ix,
-                                   Iy, PROD);
-                LK_CHECK("k_lk_tensor_products");
-                hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 3, wc,
-                                   0);
-                LK_CHECK("k_lk_boxsum");
-                hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, 3, wc,
-                                   1);
-                LK_CHECK("k_lk_boxsum");
-            }
-            for (int it = 0; it < iters && wc >= 1 && wc <= 4; it++) {
-                // the whole iteration in one kernel, (u, v) -> (nu, nv)
-                if (direct && l == 0 && it == iters - 1) {
-                    nu = du;
-                    nv = dv;
-                }
-                if (wc == 1) hipLaunchKernelGGL((k_lk_iter_fused<1>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-                if (wc == 2) hipLaunchKernelGGL((k_lk_iter_fused<2>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-                if (wc == 3) hipLaunchKernelGGL((k_lk_iter_fused<3>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-                if (wc == 4) hipLaunchKernelGGL((k_lk_iter_fused<4>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-                LK_CHECK("k_lk_iter_fused");
-                float* t = u; u = nu; nu = t;
-                t = v; v = nv; nv = t;
-            }
-            for (int it = 0; it < iters && (wc < 1 || wc > 4); it++) {
-                hipLaunchKernelGGL(k_lk_it_products, grid2d(w, h), dim3(256), 0, st, p0, p1, Ix, Iy,
-                                   u, v, w, h, PROD);
-                LK_CHECK("k_lk_it_products");
-                hipLaunchKernelGGL(k_lk_boxsum, grid2d(w, h), dim3(256), 0, st, PROD, ROWS, w, h, 2, wc, 0);
-                LK_CHECK("k_lk_boxsum");
-                hipLaunchKernelGGL(k_lk_update, grid2d(w, h), dim3(256), 0, st, ROWS, GT, w, h, wc,
-                                   det_eps, u, v);
-                LK_CHECK("k_lk_update");
-            }
-            if (l > 0) {
-                // zoom_in (zoom.cpp:61-85) of both components, times 1/zf
-                OF_TRY(launch_zoom_in_planes(u, v, nu, nv, w, h, nx[l - 1], ny[l - 1], invZ, st));
-                float* t = u; u = nu; nu = t;
-                t = v; v = nv; nv = t;
-            }
-        }
-        if (!direct) {
-            OF_HIP(hipMemcpyAsync(du, u, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
-            OF_HIP(hipMemcpyAsync(dv, v, n0 * sizeof(float), hipMemcpyDeviceToDevice, st));
-        }
-        OF_HIP(hipStreamSynchronize(st));
-        lease.drained();
-        return OF_OK;
-    };
-    if (status == OF_OK) status = run();
+    OF_TRY(lease.get()->ensure(lk_block_floats(P)));
+    const int status = lk_pyr_run(P, lk_carve(lease.get()->block, P), dI0, dI1, zf, win, iters,
+                                  det_eps, du, dv, st);
+    if (status == OF_OK) lease.drained();
     return status;
 }
 
 }  // namespace amdflow
+
+// Scratch of of_lk_pyr_compute_device_noalloc: one fixed block per GPU in
+// the code object's data, so that entry point never allocates.  Host code
+// never uses this name as an address (it is a host-side stand-in there); it
+// uses the device address that hipGetSymbolAddress returns for each device.
+constexpr size_t kLkScratchFloats = (size_t)32 << 20;   // 128 MiB
+static __device__ __align__(256) float scratch[kLkScratchFloats];
+
+namespace {
+
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[64];   // device address of `scratch` per device id
+
+// Device address of `scratch` on the current device `device`, looked up once
+// per device with that device current.
+int lk_scratch(int device, float** out)
+{
+    if (device < 0 || device >= 64) return OF_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        OF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return OF_ERR_HIP;
+        g_scratch_ptr[device] = (float*)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return OF_OK;
+}
+
+}  // namespace
 
 using namespace amdflow;
 
@@ -889,6 +986,29 @@ int of_lk_pyr_compute_device(const float* dI0, const float* dI1, int width, int 
 {
     return lk_pyr_solve(dI0, dI1, width, height, levels, zf, sigma, window, iters, det_eps, du, dv,
                         (hipStream_t)stream);
+}
+
+int of_lk_pyr_compute_device_noalloc(const float* dI0, const float* dI1, int width, int height,
+                                     int levels, float zf, float sigma, int window, int iters,
+                                     float det_eps, float* du, float* dv, void* stream)
+{
+    if (!dI0 || !dI1 || !du || !dv) return OF_ERR_ARG;
+    LkPlan P;
+    OF_TRY(lk_plan(width, height, levels, zf, sigma, window, iters, &P));
+    if (lk_block_floats(P) > kLkScratchFloats) return OF_ERR_CAPACITY;
+    int device = 0;
+    OF_HIP(hipGetDevice(&device));
+    float* base = nullptr;
+    OF_TRY(lk_scratch(device, &base));
+    const hipStream_t st = (hipStream_t)stream;
+    const int status = lk_pyr_run(P, lk_carve(base, P), dI0, dI1, zf, window, iters, det_eps, du,
+                                  dv, st);
+    if (status != OF_OK) {
+        // the next call reuses the scratch: let this one's queued work end first
+        (void)hipStreamSynchronize(st);
+        (void)hipGetLastError();
+    }
+    return status;
 }
 
 int of_lk_pyr_compute(const float* I0, const float* I1, int width, int height, int levels, float zf,
