@@ -81,6 +81,27 @@ int orc_lk_single(const uint8_t *frames, int T, int iframe, int nx, int ny, int 
 int orc_lk_pyr_compute(const float *I0, const float *I1, int width, int height, int levels,
                        float zf, float sigma, int win, int iters, float det_eps, float *u, float *v);
 
+/* FP64 oracle (oracle64.c): the same algorithms computed in double. */
+int o64_gaussian(double *I, int nx, int ny, double sigma);
+double o64_bicubic_at(const double *I, double uu, double vv, int nx, int ny, int border_out);
+double o64_tvl1_iteration(const double *wx, const double *wy, const double *rho_c, double *u1,
+                          double *u2, double *p11, double *p12, double *p21, double *p22, int nx,
+                          int ny, double tau, double lambda, double theta, double *scratch);
+int o64_dual_tvl1(const double *I0, const double *I1, double *u1, double *u2, int nx, int ny,
+                  const orc_tvl1_params *prm, int *iters);
+int o64_tvl1_run_f32(const float *I0, const float *I1, int width, int height,
+                     const orc_tvl1_params *prm, double *u, int *iters, int *nscales_out);
+void o64_hs_downscale(const double *src, int w, int h, double *out, int nw, int nh);
+void o64_hs_upscale(const double *src, int w, int h, double *out, int nw, int nh, double scale);
+void o64_hs_jacobi(const double *du0, const double *dv0, const double *Ix, const double *Iy,
+                   const double *Iz, int w, int h, double alpha, double *du1, double *dv1);
+int o64_hs_compute(const float *I0, const float *I1, int width, int height, int stride,
+                   double alpha, int nLevels, int nWarpIters, int nSolverIters, double *u,
+                   double *v);
+int o64_lk_pyr_compute(const float *I0, const float *I1, int width, int height, int levels,
+                       double zf, double sigma, int win, int iters, double det_eps, double *u,
+                       double *v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -400,6 +400,141 @@ def lk_pyr(I0, I1, levels=3, zf=0.5, sigma=1.0, win=7, iters=5, det_eps=1e-6):
     return u, v
 
 
+# ---- FP64 oracle (oracle64.c): the same algorithms in double ---------------
+
+_dp = C.POINTER(C.c_double)
+_sig("o64_gaussian", C.c_int, [_vp, C.c_int, C.c_int, C.c_double])
+_sig("o64_bicubic_at", C.c_double, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int])
+_sig("o64_tvl1_iteration", C.c_double, [_vp] * 9 + [C.c_int, C.c_int, C.c_double, C.c_double,
+                                                    C.c_double, _vp])
+_sig("o64_dual_tvl1", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _ip])
+_sig("o64_tvl1_run_f32", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _ip, _ip])
+_sig("o64_hs_downscale", None, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int])
+_sig("o64_hs_upscale", None, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_double])
+_sig("o64_hs_jacobi", None, [_vp] * 5 + [C.c_int, C.c_int, C.c_double, _vp, _vp])
+_sig("o64_hs_compute", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.c_int, C.c_int, _vp, _vp])
+_sig("o64_lk_pyr_compute", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_double, _vp, _vp])
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gaussian64(img, sigma: float) -> np.ndarray:
+    a = _f64(img).copy()
+    h, w = a.shape
+    st = _lib.o64_gaussian(_p(a), w, h, sigma)
+    if st:
+        raise ValueError(f"gaussian64: status {st}")
+    return a
+
+
+def bicubic_at64(img, uu, vv, border_out: bool) -> float:
+    a = _f64(img)
+    h, w = a.shape
+    return _lib.o64_bicubic_at(_p(a), uu, vv, w, h, int(border_out))
+
+
+def tvl1_iteration64(wx, wy, rho_c, u1, u2, p11, p12, p21, p22, tau=0.25, lambda_=0.15,
+                     theta=0.3):
+    """One primal-dual iteration in double; returns the new (u1, u2, p11,
+    p12, p21, p22) and the error sum (du1^2 + du2^2)."""
+    wx, wy, rho_c = _f64(wx), _f64(wy), _f64(rho_c)
+    st = [_f64(x).copy() for x in (u1, u2, p11, p12, p21, p22)]
+    h, w = wx.shape
+    scratch = np.empty(4 * w * h, np.float64)
+    e = _lib.o64_tvl1_iteration(_p(wx), _p(wy), _p(rho_c), *[_p(x) for x in st], w, h, tau,
+                                lambda_, theta, _p(scratch))
+    return tuple(st), e
+
+
+def dual_tvl1_64(I0, I1, u1, u2, params: Params | None = None):
+    """One scale of TV-L1 in double from (u1, u2); returns (u1, u2, iters)."""
+    I0, I1 = _f64(I0), _f64(I1)
+    u1, u2 = _f64(u1).copy(), _f64(u2).copy()
+    h, w = I0.shape
+    p = params or default_params()
+    iters = np.zeros(p.warps, np.int32)
+    st = _lib.o64_dual_tvl1(_p(I0), _p(I1), _p(u1), _p(u2), w, h, C.byref(p),
+                            iters.ctypes.data_as(_ip))
+    if st:
+        raise ValueError(f"dual_tvl1_64: status {st}")
+    return u1, u2, iters
+
+
+def tvl1_64(I0, I1, params: Params | None = None):
+    """Multiscale TV-L1 in double of a pair of frames (uint8 or float32, read
+    exactly).  Returns (u planar [u1|u2] float64, iteration counts
+    [nscales, warps])."""
+    p = params or default_params()
+    a, b = _f32(I0), _f32(I1)
+    h, w = a.shape
+    ns = nscales(w, h, p.nscales, p.zfactor)
+    u = np.zeros(2 * w * h, np.float64)
+    iters = np.zeros(ns * p.warps, np.int32)
+    nso = C.c_int()
+    st = _lib.o64_tvl1_run_f32(_p(a), _p(b), w, h, C.byref(p), _p(u), iters.ctypes.data_as(_ip),
+                               C.byref(nso))
+    if st:
+        raise ValueError(f"tvl1_64: status {st}")
+    return u, iters.reshape(ns, p.warps)
+
+
+def hs_downscale64(src, nw, nh):
+    a = _f64(src)
+    h, w = a.shape
+    out = np.empty((nh, nw), np.float64)
+    _lib.o64_hs_downscale(_p(a), w, h, _p(out), nw, nh)
+    return out
+
+
+def hs_upscale64(src, nw, nh, scale):
+    a = _f64(src)
+    h, w = a.shape
+    out = np.empty((nh, nw), np.float64)
+    _lib.o64_hs_upscale(_p(a), w, h, _p(out), nw, nh, scale)
+    return out
+
+
+def hs_jacobi64(du, dv, Ix, Iy, Iz, alpha):
+    du, dv, Ix, Iy, Iz = (_f64(x) for x in (du, dv, Ix, Iy, Iz))
+    h, w = du.shape
+    o1, o2 = np.empty_like(du), np.empty_like(dv)
+    _lib.o64_hs_jacobi(_p(du), _p(dv), _p(Ix), _p(Iy), _p(Iz), w, h, alpha, _p(o1), _p(o2))
+    return o1, o2
+
+
+def hs_compute64(I0, I1, alpha=15.0, nLevels=1, nWarpIters=1, nSolverIters=100):
+    I0, I1 = _f32(I0), _f32(I1)
+    h, w = I0.shape
+    u, v = np.zeros((h, w), np.float64), np.zeros((h, w), np.float64)
+    st = _lib.o64_hs_compute(_p(I0), _p(I1), w, h, w, alpha, nLevels, nWarpIters, nSolverIters,
+                             _p(u), _p(v))
+    if st:
+        raise ValueError(f"hs_compute64: status {st}")
+    return u, v
+
+
+def lk_pyr64(I0, I1, levels=3, zf=0.5, sigma=1.0, win=7, iters=5, det_eps=1e-6):
+    I0, I1 = _f32(I0), _f32(I1)
+    h, w = I0.shape
+    u, v = np.zeros((h, w), np.float64), np.zeros((h, w), np.float64)
+    st = _lib.o64_lk_pyr_compute(_p(I0), _p(I1), w, h, levels, zf, sigma, win, iters, det_eps,
+                                 _p(u), _p(v))
+    if st:
+        raise ValueError(f"lk_pyr64: status {st}")
+    return u, v
+
+
+def flow_error(u, u64) -> dict:
+    """Distance of a float flow from the FP64 oracle's: max, mean and 99th
+    percentile of |u - u64| over all components, in pixels."""
+    d = np.abs(np.asarray(u, np.float64).ravel() - np.asarray(u64, np.float64).ravel())
+    return {"max": float(d.max()), "mean": float(d.mean()), "p99": float(np.percentile(d, 99))}
+
+
 # ---- the reference's own sources (oracle/_ref), when built -----------------
 
 class Ref:
