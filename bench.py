@@ -25,15 +25,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sycl-optic-flow_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+WORKLOAD = "tvl1_1920x1080"
+SEED = 2024
+BATCH_SEED0 = 3_000_000
+
+# Kernel sources (and launch-shape table) that the DRAM traffic of the
+# TV-L1 kernels depends on; a measured figure below is reported only while
+# their hash equals the one it was measured with.
+TRAFFIC_SOURCES = ["sycl-optic-flow_amd/csrc/" + f for f in (
+    "tvl1_iter.hip", "tvl1_pixel.hpp", "tvl1_launch.hpp", "device_math.hpp", "tvl1.cpp",
+    "tvl1_kernels.hip", "batch.cpp", "batch_lockstep.hip", "lockstep.hpp")] + \
+    ["sycl-optic-flow_amd/tuning/tvl1_launch_shapes.json"]
+
+
+def _sources_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 # HBM bytes per launch of the dominant kernel at this workload, from two
 # rocprofv3 counter runs of this script (FETCH_SIZE, WRITE_SIZE, each scaled
 # by the same counter on a 1 GiB copy run first in the same process:
 # --copy-too); the runs are summarised under profiles/.  Counters cannot be
 # read inside a timed run, so the value is carried here and reported only
-# when the same kernel dominates.
-MEASURED_TRAFFIC = {"k_tvl1_persist<2,8>": 1777462856}
-WORKLOAD = "tvl1_1920x1080"
-SEED = 2024
+# when the same kernel dominates and the sources are the measured ones.
+MEASURED_TRAFFIC = {"sources": None, "kernels": {}}
+# DRAM bytes of one run of the 64-pair batch, measured the same way.
+MEASURED_BATCH_TRAFFIC = {"sources": None, "pairs": 64, "seed0": BATCH_SEED0, "bytes": None}
 
 
 def parse():
@@ -44,7 +66,9 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU leg (baseline timing, parity and FP64 checks)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 oracle comparison")
     ap.add_argument("--retune", action="store_true", help="re-run the launch-shape autotuner")
     ap.add_argument("--batch-pairs", type=int, default=64,
                     help="pairs of the batched mixed-size measurement (0 = skip)")
@@ -84,41 +108,45 @@ BATCH_SIZES = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
 
 
 def batched(npairs: int, lanes: int, reps: int, workers: bool = True, shapes=None):
-    """64 pairs whose sizes are drawn uniformly (seeded) from BATCH_SIZES,
-    same generator and parameters as the 1920x1080 workload, solved by the
+    """`npairs` distinct pairs whose sizes are drawn uniformly (seeded) from
+    BATCH_SIZES, each generated on the GPU from its own seed with the
+    1920x1080 workload's generator and solved with its parameters, by the
     lockstep batch solver with launch shapes from its autotuner (and, for
-    comparison, its worker mode); two
-    generated pairs per size are cycled through.  The algorithmic bytes of
-    the batch are the sum of its pairs' (each pair solved once alone for its
-    iteration counts, which the batch reproduces exactly)."""
+    comparison, its worker mode).  The algorithmic bytes of the batch are the
+    sum of its pairs' (each pair solved once alone for its iteration counts,
+    which the batch must reproduce exactly).  Keeps host copies of the first
+    pair of each size and its batch flow for the FP64 check."""
     import numpy as np
     import torch
 
-    from amdflow import TV_L1, synth
+    from amdflow import TV_L1
     from amdflow.batch import TVL1Batch
+    from amdflow.synth import DevicePairs
 
     rng = np.random.default_rng(SEED + 1)
     pick = rng.integers(0, len(BATCH_SIZES), npairs)
-    cache, alg = {}, {}
-    d0, d1, dU, ws, hs, keys = [], [], [], [], [], []
+    gens, solvers = {}, {}
+    d0, d1, dU, ws, hs, single = [], [], [], [], [], []
     for k, s in enumerate(pick):
         w, h = BATCH_SIZES[int(s)]
-        key = (w, h, k % 2)
-        if key not in cache:
-            I0, I1, _, _ = synth.make_pair(w, h, 20.0, SEED + 7 * len(cache), bumps=True)
-            cache[key] = (torch.from_numpy(synth.quantize(I0).reshape(-1)).cuda(),
-                          torch.from_numpy(synth.quantize(I1).reshape(-1)).cuda())
-        a, b = cache[key]
-        d0.append(a.data_ptr()); d1.append(b.data_ptr())
+        if (w, h) not in gens:
+            gens[(w, h)] = DevicePairs(w, h, 20.0, bumps=True)
+            solvers[(w, h)] = TV_L1(w, h, nscales=5)
+        a, b, _ = gens[(w, h)].generate(BATCH_SEED0 + k)
+        d0.append(a.clone()); d1.append(b.clone())
         dU.append(torch.empty(2 * w * h, dtype=torch.float32, device="cuda"))
-        ws.append(w); hs.append(h); keys.append(key)
-    for key, (a, b) in cache.items():
-        tv = TV_L1(key[0], key[1], nscales=5)
-        out = torch.empty(2 * key[0] * key[1], dtype=torch.float32, device="cuda")
-        tv.run_device_u8(a.data_ptr(), b.data_ptr(), out.data_ptr())
+        ws.append(w); hs.append(h)
+    torch.cuda.synchronize()
+    for k in range(npairs):
+        tv = solvers[(ws[k], hs[k])]
+        out = torch.empty(2 * ws[k] * hs[k], dtype=torch.float32, device="cuda")
+        tv.run_device_u8(d0[k].data_ptr(), d1[k].data_ptr(), out.data_ptr())
         st = tv.stats()
-        alg[key] = (st["algorithmic_bytes"], int(tv.iterations().sum()))
+        single.append((st["algorithmic_bytes"], int(tv.iterations().sum())))
+    for tv in solvers.values():
         tv.close()
+    p0 = [t.data_ptr() for t in d0]
+    p1 = [t.data_ptr() for t in d1]
     up = [t.data_ptr() for t in dU]
 
     tune = {}
@@ -134,37 +162,58 @@ def batched(npairs: int, lanes: int, reps: int, workers: bool = True, shapes=Non
                                     "split_pixels": 200000}, source="--batch-shapes")
             elif mode == "lockstep":
                 # launch shapes from the batch autotuner (untimed)
-                tuned, table, best = b.autotune(ws, hs, d0, d1, up, reps=2)
+                tuned, table, best = b.autotune(ws, hs, p0, p1, up, reps=2)
                 tune.update(shapes={"large_scales": list(tuned[0]), "small_scales": list(tuned[1]),
                                     "split_pixels": 200000},
                             uniform_ms={f"{k}x{r}": round(v, 2) for (k, r), v in table.items()},
                             chosen_ms=round(best, 2))
-            b.run_device_u8(ws, hs, d0, d1, up)                 # warm-up (handles, tables)
+            b.run_device_u8(ws, hs, p0, p1, up)                 # warm-up (handles, tables)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
-                its = b.run_device_u8(ws, hs, d0, d1, up)
+                its = b.run_device_u8(ws, hs, p0, p1, up)
             torch.cuda.synchronize()
             return (time.perf_counter() - t0) / reps, its
 
     dt, its = timed("lockstep", lanes)
     dt_w, its_w = timed("workers", 4) if workers else (None, its)
-    same = all(int(its[i]) == alg[k][1] for i, k in enumerate(keys)) and bool(np.array_equal(its, its_w))
-    B = sum(alg[k][0] for k in keys)
+    same = all(int(its[i]) == single[i][1] for i in range(npairs)) and bool(np.array_equal(its, its_w))
+    B = sum(x[0] for x in single)
     mpx = sum(w * h for w, h in zip(ws, hs)) / 1e6
-    return {"workload": f"tvl1_batch{npairs}_mixed", "pairs": npairs, "mode": "lockstep",
-            "lanes": lanes, "sizes": [f"{w}x{h}" for w, h in BATCH_SIZES],
-            "pair_counts": [int((pick == i).sum()) for i in range(len(BATCH_SIZES))],
-            "ms_per_batch": round(dt * 1e3, 2), "pairs_per_s": round(npairs / dt, 2),
-            "mpixel_per_s": round(mpx / dt, 1), "iterations_total": int(np.sum(its)),
-            "algorithmic_bytes": B, "algorithmic_GBps": round(B / dt / 1e9, 1),
-            "hbm_roofline_frac": round(B / dt / 8e12, 4),
-            "workers_mode": None if dt_w is None else {
-                "streams": 4, "ms_per_batch": round(dt_w * 1e3, 2),
-                "hbm_roofline_frac": round(B / dt_w / 8e12, 4)},
-            "iterations_match_single_solves": same,
-            "launch_shapes": tune,
-            "data": "synthetic, 2 generated pairs per size cycled through the batch"}
+    # every mode writes the same bits into dU: the flows of the last run
+    kept = {}
+    for k in range(npairs):
+        key = f"{ws[k]}x{hs[k]}"
+        if key not in kept:
+            kept[key] = {"I0": d0[k].cpu().numpy().reshape(hs[k], ws[k]),
+                         "I1": d1[k].cpu().numpy().reshape(hs[k], ws[k]),
+                         "flow": dU[k].cpu().numpy(), "iterations": int(its[k])}
+    rec = {"workload": f"tvl1_batch{npairs}_mixed", "pairs": npairs, "mode": "lockstep",
+           "lanes": lanes, "sizes": [f"{w}x{h}" for w, h in BATCH_SIZES],
+           "pair_counts": [int((pick == i).sum()) for i in range(len(BATCH_SIZES))],
+           "ms_per_batch": round(dt * 1e3, 2), "pairs_per_s": round(npairs / dt, 2),
+           "mpixel_per_s": round(mpx / dt, 1), "iterations_total": int(np.sum(its)),
+           "algorithmic_bytes": B, "algorithmic_GBps": round(B / dt / 1e9, 1),
+           "hbm_roofline_frac": round(B / dt / 8e12, 4)}
+    # DRAM bytes of the same batch from rocprofv3 counter runs (FETCH_SIZE +
+    # WRITE_SIZE, copy-calibrated), reported while the kernel sources are the
+    # ones measured
+    m = MEASURED_BATCH_TRAFFIC
+    if m and m["sources"] == _sources_hash() and m["pairs"] == npairs and m["seed0"] == BATCH_SEED0:
+        rec["measured_dram_bytes"] = m["bytes"]
+        rec["measured_dram_GBps"] = round(m["bytes"] / dt / 1e9, 1)
+        rec["measured_dram_frac"] = round(m["bytes"] / dt / 8e12, 4)
+    else:
+        rec["measured_dram_bytes"] = None
+    rec.update({
+        "workers_mode": None if dt_w is None else {
+            "streams": 4, "ms_per_batch": round(dt_w * 1e3, 2),
+            "hbm_roofline_frac": round(B / dt_w / 8e12, 4)},
+        "iterations_match_single_solves": same,
+        "launch_shapes": tune,
+        "data": f"synthetic, {npairs} distinct pairs generated on the GPU (seeds "
+                f"{BATCH_SEED0}..{BATCH_SEED0 + npairs - 1})"})
+    return rec, kept
 
 
 def _cores() -> int:
@@ -227,6 +276,8 @@ def other_paths(cpu_budget: float):
     rec["bitwise_equal_to_oracle"] = bool(np.array_equal(du.cpu().numpy(), u_o.reshape(-1))
                                           and np.array_equal(dv.cpu().numpy(), v_o.reshape(-1)))
     out["hs"] = rec
+    kept = {"hs": {"I0": I0, "I1": I1, "u": du.cpu().numpy().reshape(H, W),
+                   "v": dv.cpu().numpy().reshape(H, W)}}
 
     cfg = synth.CONFIGS["lk_1280x720"]
     W, H = cfg["width"], cfg["height"]
@@ -247,6 +298,8 @@ def other_paths(cpu_budget: float):
     rec["bitwise_equal_to_oracle"] = bool(np.array_equal(du.cpu().numpy(), u_o.reshape(-1))
                                           and np.array_equal(dv.cpu().numpy(), v_o.reshape(-1)))
     out["lk"] = rec
+    kept["lk"] = {"I0": I0, "I1": I1, "u": du.cpu().numpy().reshape(H, W),
+                  "v": dv.cpu().numpy().reshape(H, W)}
 
     # the reference's own LK interface (lucaskanade.cuh:8-11) as its application
     # drives it (src/lk/cuda/main.cu:132-160: spatial filter 3, temporal 2,
@@ -282,7 +335,7 @@ def other_paths(cpu_budget: float):
     rec["bitwise_equal_to_oracle"] = bool(np.array_equal(Vx, ox.reshape(-1)) and np.array_equal(Vy, oy.reshape(-1)))
     lk.close()
     out["lk_stream"] = rec
-    return out
+    return out, kept
 
 
 def cnn_mfma(net, batch: int, h: int, w: int, reps: int = 20):
@@ -317,9 +370,10 @@ def warm_start(npairs: int):
     from amdflow import cnn
     from amdflow.warmstart import evaluate
     if not os.path.exists(cnn.WEIGHTS):
-        return {"skipped": "no trained weights at " + os.path.relpath(cnn.WEIGHTS, ROOT)}
+        return {"skipped": "no trained weights at " + os.path.relpath(cnn.WEIGHTS, ROOT)}, None
     net = cnn.WarmStartNet.pretrained()
-    res = evaluate(net, npairs=npairs)
+    res = evaluate(net, npairs=npairs, keep_first=True)
+    first = res.pop("first_pair")
     res["workload"] = f"tvl1_warm_1920x1080_batch{npairs}"
     # the setting whose mean EPE against the plain flow is within 0.05 px
     # (the metric's bar): seed 240x135 and keep all 5 warpings
@@ -330,7 +384,7 @@ def warm_start(npairs: int):
                                                 "fallbacks") if k in tight}
     res["network"] = {"layers": f"conv3x3 5-32, {net.hidden}x 32-32, 32-2", "params": net.num_params}
     res["mfma"] = cnn_mfma(net, npairs, 68, 120)
-    return res
+    return res, first
 
 
 def _cpu_model() -> str:
@@ -379,6 +433,65 @@ def cpu_baseline(q0, q1, budget_s: float):
                              "sample": f"median of {runs1} solves, 1 thread"},
            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
     return rec, u, it
+
+
+def float_sum_replay(q0, q1, iters_double):
+    """The reference sums TV-L1's stopping error as a float; the GPU (and the
+    oracle by default) in double in a fixed order.  Replays the 1920x1080
+    solve with the reference's float sum in index order (its one-thread
+    order) and counts the warpings whose iteration count differs."""
+    import numpy as np
+    import oracle as O  # CPU checker (path set by cpu_baseline)
+    O.set_error_mode(O.ERROR_FLOAT_SEQUENTIAL)
+    try:
+        uf, itf = O.tvl1(q0, q1, O.default_params(nscales=5))
+    finally:
+        O.set_error_mode(O.ERROR_DOUBLE_ROWS)
+    return {"warpings": int(itf.size),
+            "decisions_differing": int(np.sum(itf != iters_double)),
+            "iterations_total": {"double_sum": int(iters_double.sum()), "float_sum": int(itf.sum())}}
+
+
+def fp64_error(kept):
+    """Distance of each workload's GPU flow from the FP64 oracle (the same
+    algorithm in double): max, mean and 99th percentile of |u - u_fp64| in
+    pixels over both components, and the TV-L1 iteration counts of both."""
+    import numpy as np
+    import oracle as O  # CPU checker (path set by cpu_baseline)
+    out = {}
+    t0 = time.perf_counter()
+    k = kept["tvl1_1920x1080"]
+    p = O.default_params(nscales=5)
+    u64, it64 = O.tvl1_64(k["I0"], k["I1"], p)
+    out["tvl1_1920x1080"] = {**O.flow_error(k["flow"], u64),
+                             "iterations": {"gpu": int(k["iterations"].sum()),
+                                            "fp64": int(it64.sum())}}
+    if "batch" in kept:
+        per = {}
+        for size, b in kept["batch"].items():
+            u64, it64 = O.tvl1_64(b["I0"], b["I1"], p)
+            per[size] = {**O.flow_error(b["flow"], u64),
+                         "iterations": {"gpu": b["iterations"], "fp64": int(it64.sum())}}
+        out["tvl1_batch64_mixed"] = {"pairs_checked": "first pair of each size", "per_size": per}
+    if "warm" in kept and kept["warm"]:
+        w = kept["warm"]
+        u64, it64 = O.tvl1_64(w["I0"], w["I1"], p)
+        out["tvl1_warm_1920x1080"] = {
+            "pairs_checked": 1,
+            "warm": O.flow_error(w["warm"], u64), "plain": O.flow_error(w["plain"], u64),
+            "iterations": {"plain_gpu": w["iterations_plain"], "fp64": int(it64.sum())}}
+    if "other" in kept:
+        o = kept["other"]
+        h = o["hs"]
+        u64, v64 = O.hs_compute64(h["I0"], h["I1"], 15.0, 1, 1, 100)
+        out["hs_640x480"] = O.flow_error(np.stack([h["u"], h["v"]]), np.stack([u64, v64]))
+        l = o["lk"]
+        u64, v64 = O.lk_pyr64(l["I0"], l["I1"], levels=3, zf=0.5, sigma=1.0, win=7, iters=5,
+                              det_eps=1e-6)
+        out["lk_1280x720"] = O.flow_error(np.stack([l["u"], l["v"]]), np.stack([u64, v64]))
+    out["unit"] = "px"
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 def main():
@@ -447,6 +560,25 @@ def main():
         tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
     prof = tv.profile_results()
     tv.profile(False)
+    # per-scale device time (events around each scale), separate pass
+    tv.scale_timing(True)
+    for _ in range(max(1, args.profile_steps)):
+        tv.run_device_u8(d0.data_ptr(), d1.data_ptr(), dU.data_ptr(), sp)
+    scale_ms, scale_runs = tv.scale_times()
+    tv.scale_timing(False)
+    per_scale = []
+    for s_, ms in enumerate(scale_ms):
+        nx_, ny_ = tv.scale_size(s_)
+        n_ = nx_ * ny_
+        pi = int(iters[s_].sum()) * n_
+        # the scale's share of B_alg: iterations, warpings, gradient + zoom_in
+        # + pyramid, and the finest scale's normalisation and pre-smoothing
+        b = 60.0 * pi + 36.0 * 5 * n_ + 64.0 * n_ + (48.0 * n_ if s_ == 0 else 0.0)
+        t_ = ms / scale_runs / 1e3
+        per_scale.append({"scale": s_, "size": [nx_, ny_], "ms": round(ms / scale_runs, 4),
+                          "iterations": int(iters[s_].sum()), "pixel_iterations": pi,
+                          "algorithmic_GBps": round(b / t_ / 1e9, 1),
+                          "hbm_roofline_frac": round(b / t_ / 1e9 / HBM_PEAK_GBPS, 4)})
     dom = max(prof, key=lambda p: p["milliseconds"])
     per_launch_bytes = dom["algorithmic_bytes"] / dom["launches"]
     avg_launch_s = dom["milliseconds"] / dom["launches"] / 1e3
@@ -462,12 +594,17 @@ def main():
                                       / (1e3 * dt / args.steps), 3),
     }
     # keep "traffic" right after "frac" in the printed object
-    traffic = MEASURED_TRAFFIC.get(roofline["kernel"])
+    traffic = MEASURED_TRAFFIC["kernels"].get(roofline["kernel"]) \
+        if MEASURED_TRAFFIC["sources"] == _sources_hash() else None
     roofline = {**{k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
                 "traffic": traffic,
                 **{k: v for k, v in roofline.items() if k not in ("bound", "achieved", "peak", "unit", "frac")}}
     if traffic is not None:
         roofline["traffic_note"] = "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+        roofline["measured_GBps"] = round(traffic / avg_launch_s / 1e9, 1)
+        roofline["measured_frac"] = round(traffic / avg_launch_s / 1e9 / HBM_PEAK_GBPS, 4)
+    else:
+        roofline["traffic_note"] = "not measured for these kernel sources"
 
     flow = dU.cpu().numpy()
     result = {
@@ -493,18 +630,22 @@ def main():
         "launch_shapes_per_scale": [list(s) for s in shapes],
         "launch_shapes_source": "autotuned in this run" if tuned_now else "tuning table",
         "roofline": roofline,
+        "per_scale": per_scale,
     }
     # the solver handle is done; closing it also releases the stream the
     # autotuner gave it, which would otherwise share a hardware queue with a
     # batch lane
     tv.close()
+    kept = {"tvl1_1920x1080": {"I0": q0, "I1": q1, "flow": flow, "iterations": iters}}
     if args.batch_pairs > 0:
-        result["batched"] = batched(args.batch_pairs, args.batch_lanes, reps=args.batch_reps,
-                                    workers=not args.no_batch_workers, shapes=bshapes)
+        result["batched"], kept["batch"] = batched(args.batch_pairs, args.batch_lanes,
+                                                   reps=args.batch_reps,
+                                                   workers=not args.no_batch_workers,
+                                                   shapes=bshapes)
     if args.warm_pairs > 0:
-        result["warm_start"] = warm_start(args.warm_pairs)
+        result["warm_start"], kept["warm"] = warm_start(args.warm_pairs)
     if not args.no_other:
-        result["other_paths"] = other_paths(3.0 if not args.no_cpu else 0.0)
+        result["other_paths"], kept["other"] = other_paths(3.0 if not args.no_cpu else 0.0)
     if not args.no_cpu:
         rec, uo, ito = cpu_baseline(q0, q1, args.cpu_seconds)
         result["cpu_baseline"] = rec
@@ -512,7 +653,10 @@ def main():
             "max_abs_px_vs_oracle": float(np.max(np.abs(flow - uo))),
             "bitwise_equal": bool(np.array_equal(flow, uo)),
             "iterations_equal": bool(np.array_equal(iters, ito)),
+            "float_sum_replay": float_sum_replay(q0, q1, ito),
         }
+        if not args.no_fp64:
+            result["fp64_error"] = fp64_error(kept)
     print(json.dumps(result), flush=True)
 
 

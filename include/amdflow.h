@@ -175,6 +175,15 @@ int of_tvl1_profile_enable(of_tvl1 *h, int enable);
  * the number of entries. */
 int of_tvl1_profile_get(const of_tvl1 *h, of_tvl1_kernel_profile *out, int cap, int *count);
 
+/* Per-scale device time (no reference counterpart): when enabled, HIP
+ * events recorded on the solve's stream before each scale's gradient and
+ * after its prolongation time every scale of the solves that follow.
+ * Enabling (or disabling) resets the sums.  of_tvl1_scale_times writes up
+ * to `cap` summed milliseconds, finest scale first, and the number of
+ * solves timed. */
+int of_tvl1_scale_timing_enable(of_tvl1 *h, int enable);
+int of_tvl1_scale_times(const of_tvl1 *h, double *ms, int cap, int *solves);
+
 /* ------------------------------------------------------------------------
  * Batched TV-L1 (no reference counterpart): many frame pairs of mixed sizes
  * solved together.  Results are identical to solving each pair alone.

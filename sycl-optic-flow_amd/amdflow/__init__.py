@@ -113,6 +113,8 @@ SIGNATURES = {
     "of_tvl1_set_inflight": (C.c_int, [_vp, C.c_int]),
     "of_tvl1_profile_enable": (C.c_int, [_vp, C.c_int]),
     "of_tvl1_profile_get": (C.c_int, [_vp, C.POINTER(TVL1KernelProfile), C.c_int, _ip]),
+    "of_tvl1_scale_timing_enable": (C.c_int, [_vp, C.c_int]),
+    "of_tvl1_scale_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.c_int, _ip]),
     "of_tvl1_prepare_device_u8": (C.c_int, [_vp, _vp, _vp, _vp]),
     "of_tvl1_scale_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "of_tvl1_solve_prepared": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
@@ -313,6 +315,17 @@ class TV_L1:
         _check(_lib.of_tvl1_profile_get(self._h, arr, 16, C.byref(n)), "profile_get")
         return [{f: getattr(arr[i], f) for f, _ in TVL1KernelProfile._fields_}
                 for i in range(min(n.value, 16))]
+
+    def scale_timing(self, enable: bool = True) -> None:
+        """Per-scale device time of the following solves (resets the sums)."""
+        _check(_lib.of_tvl1_scale_timing_enable(self._h, int(enable)), "scale_timing_enable")
+
+    def scale_times(self) -> tuple[list[float], int]:
+        """(summed milliseconds per scale, finest first; solves timed)."""
+        arr = (C.c_double * 32)()
+        n = C.c_int()
+        _check(_lib.of_tvl1_scale_times(self._h, arr, 32, C.byref(n)), "scale_times")
+        return [arr[s] for s in range(self.nscales)], n.value
 
     def clear_launch_rules(self) -> None:
         _check(_lib.of_tvl1_set_launch_rule(self._h, -1, 1, 2, 0), "set_launch_rule")

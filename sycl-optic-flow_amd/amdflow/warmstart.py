@@ -116,10 +116,13 @@ def epe(a, b, n: int) -> float:
 
 def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0,
              first_scale: int = FIRST_SCALE, warps: int = 2, max_iterations: int = 100,
-             width: int = WIDTH, height: int = HEIGHT, cap: float = CAP, reps: int = 2):
+             width: int = WIDTH, height: int = HEIGHT, cap: float = CAP, reps: int = 2,
+             keep_first: bool = False):
     """EPE against the converged plain solve and the true motion, and pairs/s,
     of the plain solve, the warm-started solve and the same limited solve
-    from zero flow, over `npairs` held-out pairs."""
+    from zero flow, over `npairs` held-out pairs.  keep_first adds
+    "first_pair": host copies of the first pair's frames, its plain and
+    warm-started flows and the plain solve's iteration count."""
     import torch
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
@@ -142,6 +145,7 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0,
     dC = torch.empty(2 * cw * ch, dtype=torch.float32, device="cuda")
     dL = torch.empty(2 * fw * fh, dtype=torch.float32, device="cuda")
     fell, e_pred, e_zero, its = [], [], [], {k: 0 for k in kinds}
+    first_its = {}
 
     def run(kind, i):
         d0, d1, _ = pairs[i]
@@ -166,6 +170,8 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0,
         for i in range(npairs):                  # warm-up and results
             r = run(kind, i)
             its[kind] += int(tv.iterations().sum())
+            if i == 0:
+                first_its[kind] = int(tv.iterations().sum())
             if kind == "warm":
                 fell.append(r[0]); e_pred.append(r[1]); e_zero.append(r[2])
         stream.synchronize()
@@ -196,6 +202,12 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0,
         "fallbacks": int(sum(fell)),
         "mean_energy_ratio_pred_vs_zero": float(np.mean(np.asarray(e_pred) / np.asarray(e_zero))),
     }
+    if keep_first:
+        res["first_pair"] = {
+            "I0": pairs[0][0].cpu().numpy().reshape(height, width),
+            "I1": pairs[0][1].cpu().numpy().reshape(height, width),
+            "plain": outs["plain"][0].cpu().numpy(), "warm": outs["warm"][0].cpu().numpy(),
+            "iterations_plain": first_its["plain"]}
     tv.close()
     return res
 

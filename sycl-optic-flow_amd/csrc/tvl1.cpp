@@ -351,12 +351,14 @@ int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_it
         }
     }
     if (h->profiling) OF_TRY(collect_profile(h));
-    if (h->scale_timing)
-        for (int s = 0; s < h->ns; s++) {
+    if (h->scale_timing) {
+        for (int s = first; s >= 0; s--) {
             float ms = 0.0f;
             OF_HIP(hipEventElapsedTime(&ms, h->scale_ev[2 * s], h->scale_ev[2 * s + 1]));
             h->scale_ms[s] += ms;
         }
+        h->scale_solves++;
+    }
     h->solved = true;
     return OF_OK;
 }
@@ -891,6 +893,29 @@ int of_tvl1_profile_enable(of_tvl1* h, int enable)
         h->timed.clear();
         return OF_OK;
     });
+}
+
+int of_tvl1_scale_timing_enable(of_tvl1* h, int enable)
+{
+    return guarded([&]() -> int {
+        if (!h) return OF_ERR_ARG;
+        if (enable && h->scale_ev.empty()) {
+            h->scale_ev.resize(2 * kMaxScales);
+            for (hipEvent_t& e : h->scale_ev) OF_HIP(hipEventCreate(&e));
+        }
+        h->scale_timing = enable != 0;
+        for (double& m : h->scale_ms) m = 0.0;
+        h->scale_solves = 0;
+        return OF_OK;
+    });
+}
+
+int of_tvl1_scale_times(const of_tvl1* h, double* ms, int cap, int* solves)
+{
+    if (!h || (!ms && cap > 0)) return OF_ERR_ARG;
+    for (int s = 0; s < h->ns && s < cap; s++) ms[s] = h->scale_ms[s];
+    if (solves) *solves = h->scale_solves;
+    return OF_OK;
 }
 
 int of_tvl1_profile_get(const of_tvl1* h, of_tvl1_kernel_profile* out, int cap, int* count)

@@ -83,9 +83,10 @@ struct of_tvl1 {
     std::vector<of_tvl1_kernel_profile> prof;
     int scale_K[amdflow::kMaxScales]{}, scale_R[amdflow::kMaxScales]{}, scale_P[amdflow::kMaxScales]{};
 
-    // per-scale event timing (autotuner)
+    // per-scale event timing (autotuner, of_tvl1_scale_timing_enable)
     bool scale_timing = false;
     std::vector<hipEvent_t> scale_ev;
     double scale_ms[amdflow::kMaxScales]{};
+    int scale_solves = 0;
 };
 

@@ -159,3 +159,40 @@ def test_lockstep_edge_shapes_match_oracle():
         uo, ito = O.tvl1(q0, q1)
         assert it[k] == int(ito.sum()), k
         assert np.array_equal(flows[k], uo), k
+
+
+def test_lockstep_64_pair_mixed_batch_matches_oracle():
+    """The batched workload at full size: 64 distinct pairs, sizes drawn
+    uniformly from 320x240, 640x480, 1024x436, 1280x720 and 1920x1080, motion
+    up to 20 px, 5 scales, default parameters, three lockstep lanes.  Every
+    pair's iteration counts per (scale, warping) total and its flow equal the
+    oracle's."""
+    import time
+
+    from amdflow.batch import TVL1Batch
+    all_sizes = [(320, 240), (640, 480), (1024, 436), (1280, 720), (1920, 1080)]
+    rng = np.random.default_rng(64)
+    sizes = [all_sizes[int(i)] for i in rng.integers(0, len(all_sizes), 64)]
+    # frames generated on the GPU (of_synth_pair), one seed per pair
+    import torch
+    from amdflow.synth import DevicePairs
+    gens = {s: DevicePairs(s[0], s[1], 20.0, bumps=True) for s in all_sizes}
+    d0, d1 = [], []
+    for k, s in enumerate(sizes):
+        a, b_, _ = gens[s].generate(5000 + k)
+        d0.append(a.clone())
+        d1.append(b_.clone())
+    torch.cuda.synchronize()
+    frames = [(a.cpu().numpy().reshape(h, w), b_.cpu().numpy().reshape(h, w))
+              for a, b_, (w, h) in zip(d0, d1, sizes)]
+    with TVL1Batch(streams=3, nscales=5) as b:
+        it, flows = _run(b, sizes, d0, d1)
+    print(f"64-pair batch solved, {int(np.sum(it))} iterations; checking against the oracle",
+          flush=True)
+    t0 = time.perf_counter()
+    p = O.default_params(nscales=5)
+    for k, (q0, q1) in enumerate(frames):
+        uo, ito = O.tvl1(q0, q1, p)
+        assert it[k] == int(ito.sum()), (k, sizes[k], it[k], int(ito.sum()))
+        assert np.array_equal(flows[k], uo), (k, sizes[k])
+    print(f"oracle: 64 pairs in {time.perf_counter() - t0:.1f} s", flush=True)
