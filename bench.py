@@ -356,12 +356,22 @@ def cnn_mfma(net, batch: int, h: int, w: int, reps: int = 20):
         e1.record(s)
     s.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    macs = sum(9 * ci * co for ci, co in [(5, 32)] + [(32, 32)] * net.hidden + [(32, 2)])
+    layers = [(5, 32)] + [(32, 32)] * net.hidden + [(32, 2)]
+    macs = sum(9 * ci * co for ci, co in layers)
     flops = 2.0 * macs * batch * h * w
     tflops = flops / (ms * 1e-3) / 1e12
+    # the network's own ceiling: each layer reads its fp32 NHWC input and
+    # writes its output once (ci + co floats per pixel), so the whole forward
+    # moves 4 * sum(ci + co) bytes per pixel; min(MFMA peak, intensity x HBM)
+    bytes_px = 4.0 * sum(ci + co for ci, co in layers)
+    intensity = 2.0 * macs / bytes_px
+    ceiling = min(2500.0, intensity * HBM_PEAK_GBPS / 1e3)
     return {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
             "frac": round(tflops / 2500.0, 5), "flops_per_forward": flops,
-            "ms_per_forward": round(ms, 4), "batch": batch, "shape": [h, w, 5], "dtype": "bf16"}
+            "ms_per_forward": round(ms, 4), "batch": batch, "shape": [h, w, 5], "dtype": "bf16",
+            "flop_per_byte": round(intensity, 1), "ceiling": round(ceiling, 1),
+            "frac_of_ceiling": round(tflops / ceiling, 4),
+            "ceiling_note": "min(bf16 MFMA peak, FLOP/byte x 8 TB/s) with fp32 activations in HBM"}
 
 
 def warm_start(npairs: int):

@@ -266,6 +266,14 @@ int of_tvl1_scale_images(const of_tvl1 *h, int s, const float **dI0s, const floa
  * floats).  May be called again on the same prepared pair. */
 int of_tvl1_solve_prepared(of_tvl1 *h, const float *dUinit, int first_scale, int warps,
                            int max_iterations, float *dU, void *stream);
+/* Phase 2 with limits per scale (no reference counterpart; warm-start
+ * schedules): warps[s] warpings (<= the handle's) and max_iterations[s]
+ * iterations per warping at scale s, arrays of of_tvl1_nscales entries,
+ * finest scale first; 0 selects the handle's value.  Otherwise as
+ * of_tvl1_solve_prepared. */
+int of_tvl1_solve_prepared_schedule(of_tvl1 *h, const float *dUinit, int first_scale,
+                                    const int *warps, const int *max_iterations, float *dU,
+                                    void *stream);
 /* Carries a planar flow from scale from_scale to the finer scale to_scale
  * the way the solve does between scales (zoom_in, zoom.cpp:61-85, times
  * 1/zfactor per scale, tvl1.cpp:167-172). */

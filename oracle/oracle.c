@@ -605,7 +605,7 @@ void orc_tvl1_default_params(orc_tvl1_params *p)
  * coarser scales are not solved. */
 static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
                     const orc_tvl1_params *prm, const float *uinit, int first, float *u,
-                    int *iters, int *nscales_out)
+                    int *iters, int *nscales_out, const int *warps_s, const int *max_s)
 {
     const int ns = orc_tvl1_nscales(width, height, prm->nscales, prm->zfactor);
     if (nscales_out) *nscales_out = ns;
@@ -643,8 +643,16 @@ static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
             u2s[first][i] = uinit ? uinit[nc + i] : 0.0f;
         }
         const float invZ = 1 / prm->zfactor;
+        /* per-scale limits (warm-start schedules): a copy of the parameters
+         * with that scale's warpings and iteration cap */
+        orc_tvl1_params ps[64];
+        for (int s = 0; s < ns; s++) {
+            ps[s] = *prm;
+            if (warps_s && warps_s[s] > 0) ps[s].warps = warps_s[s];
+            if (max_s && max_s[s] > 0) ps[s].max_iterations = max_s[s];
+        }
         for (int s = first; s > 0 && !st; s--) {
-            st = orc_dual_tvl1(I0s[s], I1s[s], u1s[s], u2s[s], nx[s], ny[s], prm,
+            st = orc_dual_tvl1(I0s[s], I1s[s], u1s[s], u2s[s], nx[s], ny[s], &ps[s],
                                iters ? iters + (size_t)s * prm->warps : NULL);
             orc_zoom_in(u1s[s], u1s[s - 1], nx[s], ny[s], nx[s - 1], ny[s - 1]);
             orc_zoom_in(u2s[s], u2s[s - 1], nx[s], ny[s], nx[s - 1], ny[s - 1]);
@@ -654,7 +662,7 @@ static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
                 u2s[s - 1][i] = invZ * u2s[s - 1][i];
             }
         }
-        if (!st) st = orc_dual_tvl1(I0s[0], I1s[0], u1s[0], u2s[0], nx[0], ny[0], prm, iters);
+        if (!st) st = orc_dual_tvl1(I0s[0], I1s[0], u1s[0], u2s[0], nx[0], ny[0], &ps[0], iters);
     }
     for (int s = 0; s < ns; s++) {
         free(I0s[s]);
@@ -668,7 +676,7 @@ static int tvl1_run(const float *I0in, const float *I1in, int width, int height,
 int orc_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height,
                      const orc_tvl1_params *prm, float *u, int *iters, int *nscales_out)
 {
-    return tvl1_run(I0in, I1in, width, height, prm, NULL, -1, u, iters, nscales_out);
+    return tvl1_run(I0in, I1in, width, height, prm, NULL, -1, u, iters, nscales_out, NULL, NULL);
 }
 
 /* tvl1.cpp:118-123 then the multiscale driver, from `uinit` (or zero). */
@@ -684,7 +692,31 @@ int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int he
         f0[i] = (float)I0[i];
         f1[i] = (float)I1[i];
     }
-    const int st = tvl1_run(f0, f1, width, height, prm, uinit, first, u, iters, nscales_out);
+    const int st = tvl1_run(f0, f1, width, height, prm, uinit, first, u, iters, nscales_out, NULL,
+                            NULL);
+    free(f0);
+    free(f1);
+    return st;
+}
+
+/* The same with per-scale limits (no reference counterpart): warps_s[s]
+ * warpings and max_s[s] iterations per warping at scale s (entries <= 0
+ * keep prm's).  Iteration counts keep prm->warps slots per scale. */
+int orc_tvl1_run_schedule_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
+                             const orc_tvl1_params *prm, const float *uinit, int first,
+                             const int *warps_s, const int *max_s, float *u, int *iters,
+                             int *nscales_out)
+{
+    const size_t size = (size_t)width * height;
+    float *f0 = (float *)malloc(sizeof(float) * size);
+    float *f1 = (float *)malloc(sizeof(float) * size);
+    if (!f0 || !f1) { free(f0); free(f1); return ORC_ERR_ALLOC; }
+    for (size_t i = 0; i < size; i++) {
+        f0[i] = (float)I0[i];
+        f1[i] = (float)I1[i];
+    }
+    const int st = tvl1_run(f0, f1, width, height, prm, uinit, first, u, iters, nscales_out,
+                            warps_s, max_s);
     free(f0);
     free(f1);
     return st;

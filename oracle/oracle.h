@@ -60,6 +60,10 @@ int orc_tvl1_run_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
 int orc_tvl1_run_init_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
                          const orc_tvl1_params *prm, const float *uinit, int first, float *u,
                          int *iters, int *nscales_out);
+int orc_tvl1_run_schedule_u8(const uint8_t *I0, const uint8_t *I1, int width, int height,
+                             const orc_tvl1_params *prm, const float *uinit, int first,
+                             const int *warps_s, const int *max_s, float *u, int *iters,
+                             int *nscales_out);
 double orc_tvl1_energy(const float *I0, const float *I1, const float *u1, const float *u2, int nx,
                        int ny, float lambda, double *data_out, double *tv_out);
 

@@ -95,6 +95,8 @@ _sig("orc_tvl1_run_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params),
 _sig("orc_tvl1_run_f32", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, _ip, _ip])
 _sig("orc_tvl1_run_init_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp, C.c_int,
                                        _vp, _ip, _ip])
+_sig("orc_tvl1_run_schedule_u8", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(Params), _vp,
+                                           C.c_int, _vp, _vp, _vp, _ip, _ip])
 _sig("orc_tvl1_energy", C.c_double, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)])
 _sig("orc_bilinear_at", C.c_float, [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float])
@@ -303,6 +305,32 @@ def tvl1_init(I0, I1, uinit, params: Params | None = None, first: int = -1):
                                    int(first), _p(u), iters.ctypes.data_as(_ip), C.byref(nso))
     if st:
         raise ValueError(f"tvl1_init: status {st}")
+    return u, iters.reshape(ns, p.warps)
+
+
+def tvl1_schedule(I0, I1, warps, max_iterations, uinit=None, params: Params | None = None,
+                  first: int = -1):
+    """tvl1_init with per-scale limits: warps[s] warpings and
+    max_iterations[s] iterations per warping at scale s (finest first; 0 =
+    the parameters' value).  Returns (u, iteration counts [nscales, warps])."""
+    p = params or default_params()
+    I0 = np.ascontiguousarray(I0, dtype=np.uint8)
+    h, w = I0.shape
+    ns = nscales(w, h, p.nscales, p.zfactor)
+    u = np.zeros(2 * w * h, np.float32)
+    iters = np.zeros(ns * p.warps, np.int32)
+    ws = np.zeros(64, np.int32)
+    ms = np.zeros(64, np.int32)
+    ws[:len(warps)] = warps
+    ms[:len(max_iterations)] = max_iterations
+    nso = C.c_int()
+    b = np.ascontiguousarray(I1, dtype=np.uint8)
+    ui = None if uinit is None else _f32(uinit)
+    st = _lib.orc_tvl1_run_schedule_u8(_p(I0), _p(b), w, h, C.byref(p),
+                                       None if ui is None else _p(ui), first, _p(ws), _p(ms), _p(u),
+                                       iters.ctypes.data_as(_ip), C.byref(nso))
+    if st:
+        raise ValueError(f"tvl1_schedule: status {st}")
     return u, iters.reshape(ns, p.warps)
 
 

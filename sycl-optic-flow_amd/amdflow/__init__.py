@@ -118,6 +118,7 @@ SIGNATURES = {
     "of_tvl1_prepare_device_u8": (C.c_int, [_vp, _vp, _vp, _vp]),
     "of_tvl1_scale_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "of_tvl1_solve_prepared": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "of_tvl1_solve_prepared_schedule": (C.c_int, [_vp, _vp, C.c_int, _ip, _ip, _vp, _vp]),
     "of_tvl1_set_init_flow": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "of_tvl1_lift_flow": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, _vp]),
     "of_tvl1_flow_to_scale": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
@@ -362,6 +363,20 @@ class TV_L1:
         the device flow d_Uinit at first_scale (0: zero flow)."""
         _check(_lib.of_tvl1_solve_prepared(self._h, d_Uinit or None, int(first_scale), int(warps),
                                            int(max_iterations), d_U, stream or None), "solve_prepared")
+
+    def solve_prepared_schedule(self, d_U: int, warps, max_iterations, d_Uinit: int = 0,
+                                first_scale: int = -1, stream: int = 0) -> None:
+        """solve_prepared with per-scale limits: warps[s] warpings and
+        max_iterations[s] iterations per warping at scale s (finest first;
+        0 = the handle's value)."""
+        ns = self.nscales
+        w = np.zeros(ns, np.int32)
+        m = np.zeros(ns, np.int32)
+        w[:len(warps)] = warps
+        m[:len(max_iterations)] = max_iterations
+        _check(_lib.of_tvl1_solve_prepared_schedule(self._h, d_Uinit or None, int(first_scale),
+                                                    w.ctypes.data_as(_ip), m.ctypes.data_as(_ip),
+                                                    d_U, stream or None), "solve_prepared_schedule")
 
     def lift_flow(self, from_scale: int, d_from: int, to_scale: int, d_to: int, stream: int = 0) -> None:
         _check(_lib.of_tvl1_lift_flow(self._h, int(from_scale), d_from, int(to_scale), d_to,

@@ -23,6 +23,7 @@ give).
 
     PYTHONPATH=sycl-optic-flow_amd python -m amdflow.warmstart --pairs 20000 \
         --epochs 8 --out sycl-optic-flow_amd/weights/warmstart_c32_h5.npy
+    PYTHONPATH=sycl-optic-flow_amd python -m amdflow.warmstart --bound
 """
 from __future__ import annotations
 
@@ -212,8 +213,89 @@ def evaluate(net: WarmStartNet, npairs: int = 16, seed0: int = EVAL_SEED0,
     return res
 
 
+def schedule_bound(npairs: int = 8, first_scales=(1, 2, 3), finest_warps=(1, 2, 3, 5),
+                   coarse_warps=(1, 2), caps=(30, 100, 300), seed0: int = EVAL_SEED0,
+                   width: int = WIDTH, height: int = HEIGHT, cap: float = CAP, reps: int = 2,
+                   log=None):
+    """What any seed can gain: each held-out pair is seeded at scale s with
+    the plain solve's own final flow resampled to that scale (a perfect
+    prediction, far better than a network working at the coarsest scale),
+    then scales s .. 0 are solved with `finest_warps` warpings at scale 0,
+    `coarse_warps` at scales 1 .. s and an iteration cap per warping.  Each
+    schedule's time (frames resident; preparation included, the seed's
+    cost not) and mean EPE against the plain solve; returns the plain time,
+    every schedule and the fastest one within 0.05 px."""
+    import itertools
+
+    import torch
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    n = width * height
+    gen = DevicePairs(width, height, cap, bumps=True)
+    pairs = []
+    for i in range(npairs):
+        a, b, _ = gen.generate(seed0 + i, sp)
+        stream.synchronize()
+        pairs.append((a.clone(), b.clone()))
+    tv = _solver(width, height)
+    plain = [torch.empty(2 * n, dtype=torch.float32, device="cuda") for _ in range(npairs)]
+
+    def run_plain():
+        for i, (a, b) in enumerate(pairs):
+            tv.run_device_u8(a.data_ptr(), b.data_ptr(), plain[i].data_ptr(), sp)
+
+    run_plain()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run_plain()
+    stream.synchronize()
+    t_plain = (time.perf_counter() - t0) / (reps * npairs)
+    seeds = {}
+    for s in first_scales:
+        nx, ny = tv.scale_size(s)
+        seeds[s] = [torch.empty(2 * nx * ny, dtype=torch.float32, device="cuda")
+                    for _ in range(npairs)]
+        for i in range(npairs):
+            tv.flow_to_scale(plain[i].data_ptr(), s, seeds[s][i].data_ptr(), sp)
+    out = torch.empty(2 * n, dtype=torch.float32, device="cuda")
+    rows = []
+    for s in first_scales:
+        for w0, wc, c in itertools.product(finest_warps, coarse_warps, caps):
+            warps = [w0] + [wc] * s
+            limits = [c] * (s + 1)
+
+            def run(errs=None):
+                for i, (a, b) in enumerate(pairs):
+                    tv.prepare_device_u8(a.data_ptr(), b.data_ptr(), sp)
+                    tv.solve_prepared_schedule(out.data_ptr(), warps, limits,
+                                               seeds[s][i].data_ptr(), s, sp)
+                    if errs is not None:
+                        stream.synchronize()
+                        errs.append(epe(out, plain[i], n))
+
+            errs = []
+            run(errs)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            stream.synchronize()
+            t = (time.perf_counter() - t0) / (reps * npairs)
+            rows.append({"first_scale": s, "warps": warps, "cap": c, "ms": round(t * 1e3, 3),
+                         "speedup": round(t_plain / t, 3), "mean_epe_vs_plain": float(np.mean(errs))})
+            if log:
+                log(json.dumps(rows[-1]))
+    tv.close()
+    ok = [r for r in rows if r["mean_epe_vs_plain"] <= 0.05]
+    return {"pairs": npairs, "plain_ms": round(t_plain * 1e3, 3), "schedules": rows,
+            "fastest_within_0_05": max(ok, key=lambda r: r["speedup"]) if ok else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--bound", action="store_true",
+                    help="only the perfect-seed bound over warping schedules (schedule_bound)")
     ap.add_argument("--pairs", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=32)
@@ -225,6 +307,9 @@ def main():
     import torch
     torch.cuda.set_device(0)
     log = lambda m: print(m, flush=True)   # noqa: E731
+    if args.bound:
+        print(json.dumps(schedule_bound(log=log)), flush=True)
+        return
     X, T, hw = make_dataset(args.pairs, log=log)
     net = WarmStartNet(seed=args.seed)
     log(f"network: {net.num_params} parameters; coarsest scale {hw[1]}x{hw[0]}")

@@ -297,15 +297,18 @@ namespace {
 
 // Second phase: the coarse-to-fine solve from the prepared pyramids, with
 // `init` (planar [u1|u2] at the coarsest scale, device) as the initial flow,
-// or zero when null (tvl1.cpp:157-158); at most `warps` warpings per scale
-// and `max_iter` iterations per warping.  Waits for the result.
-int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_iter, float* dst,
-                 hipStream_t st)
+// or zero when null (tvl1.cpp:157-158); at most warps_s[s] warpings and
+// max_s[s] iterations per warping at scale s.  Waits for the result.
+int solve_levels_schedule(of_tvl1* h, const float* init, int first, const int* warps_s,
+                          const int* max_s, float* dst, hipStream_t st)
 {
     const of_tvl1_params& P = h->prm;
     const size_t N0 = (size_t)h->W * h->H;
-    h->run_warps = warps;
-    h->run_max = max_iter;
+    for (int s = 0; s < h->ns; s++) {
+        h->run_warps_s[s] = s <= first ? warps_s[s] : 0;
+        h->run_max_s[s] = max_s[s];
+    }
+    h->run_first = first;
     // every scale's current buffer back to 0 (the pyramids stay as prepared)
     OF_TRY(launch_ctl_init(h->ctl, st));
     OF_HIP(hipMemsetAsync(h->d_iters, 0, sizeof(int) * h->h_iters.size(), st));
@@ -319,6 +322,9 @@ int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_it
     for (int s = first; s >= 0; s--) {
         const int nx = h->nx[s], ny = h->ny[s];
         const size_t N = (size_t)nx * ny;
+        const int warps = warps_s[s];
+        h->run_warps = warps;
+        h->run_max = max_s[s];
         float* I0s = h->I0p + h->off[s];
         float* I1s = h->I1p + h->off[s];
         float* ub = h->ubase + 4 * h->off[s];
@@ -361,6 +367,18 @@ int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_it
     }
     h->solved = true;
     return OF_OK;
+}
+
+// The same limits at every scale.
+int solve_levels(of_tvl1* h, const float* init, int first, int warps, int max_iter, float* dst,
+                 hipStream_t st)
+{
+    int w[kMaxScales], m[kMaxScales];
+    for (int s = 0; s < kMaxScales; s++) {
+        w[s] = warps;
+        m[s] = max_iter;
+    }
+    return solve_levels_schedule(h, init, first, w, m, dst, st);
 }
 
 // One full multiscale solve with the handle's parameters and zero initial flow.
@@ -657,6 +675,25 @@ int of_tvl1_solve_prepared(of_tvl1* h, const float* dUinit, int first_scale, int
     });
 }
 
+int of_tvl1_solve_prepared_schedule(of_tvl1* h, const float* dUinit, int first_scale,
+                                    const int* warps, const int* max_iterations, float* dU,
+                                    void* stream)
+{
+    return guarded([&]() -> int {
+        if (!h || !dU || !warps || !max_iterations) return OF_ERR_ARG;
+        if (first_scale < -1 || first_scale >= h->ns) return OF_ERR_ARG;
+        if (!h->prepared) return OF_ERR_STATE;
+        if (first_scale < 0) first_scale = h->ns - 1;
+        int w[kMaxScales], m[kMaxScales];
+        for (int s = 0; s < h->ns; s++) {
+            if (warps[s] < 0 || warps[s] > h->prm.warps || max_iterations[s] < 0) return OF_ERR_ARG;
+            w[s] = warps[s] ? warps[s] : h->prm.warps;
+            m[s] = max_iterations[s] ? max_iterations[s] : h->prm.max_iterations;
+        }
+        return solve_levels_schedule(h, dUinit, first_scale, w, m, dU, (hipStream_t)stream);
+    });
+}
+
 int of_dbg_tvl1_dual(const float* I0, const float* I1, float* u1, float* u2, int width, int height,
                      const of_tvl1_params* params, int* iters)
 {
@@ -857,7 +894,10 @@ int of_tvl1_get_stats(const of_tvl1* h, of_tvl1_stats* st)
         for (int w = 0; w < h->prm.warps; w++) pi += (long long)h->h_iters[(size_t)s * h->prm.warps + w] * N;
     }
     st->pixel_iterations = pi;
-    st->pixel_warps = (long long)(h->solved ? h->run_warps : h->prm.warps) * sumN;
+    long long pw = 0;
+    for (int s = 0; s < h->ns; s++)
+        pw += (long long)(h->solved ? h->run_warps_s[s] : h->prm.warps) * h->nx[s] * h->ny[s];
+    st->pixel_warps = pw;
     st->pixels_all_scales = sumN;
     st->launches = h->launches_issued;
     st->host_launch_seconds = h->host_launch_s;

@@ -61,7 +61,9 @@ struct of_tvl1 {
     double* energy = nullptr;   // 2*kEnergyBlocks partials, then (E, data, TV)
     bool prepared = false;      // pyramids hold the last prepared pair
     bool solved = false;        // per-scale flows hold the last solve
-    int run_warps = 0, run_max = 0;
+    int run_warps = 0, run_max = 0;   // limits of the scale being solved
+    int run_warps_s[amdflow::kMaxScales]{}, run_max_s[amdflow::kMaxScales]{};   // per scale
+    int run_first = -1;               // coarsest scale solved in the last solve
     int* d_iters = nullptr;
     unsigned long long* hstatus = nullptr;   // host-mapped
     unsigned epoch = 0;

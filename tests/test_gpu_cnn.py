@@ -213,6 +213,33 @@ def test_solve_from_initial_flow_matches_oracle(warps, max_it):
     assert np.array_equal(dC.cpu().numpy(), np.concatenate([c1.ravel(), c2.ravel()]))
 
 
+@pytest.mark.parametrize("first,warps,caps", [(1, [5, 1], [300, 30]), (2, [3, 1, 2], [100, 100, 7]),
+                                              (-1, [1, 2, 3, 4, 5], [0, 9, 0, 50, 0])])
+def test_solve_schedule_matches_oracle(first, warps, caps):
+    """Per-scale warpings and iteration caps (of_tvl1_solve_prepared_schedule)
+    from a given flow at `first` are bit-exact with the oracle's schedule
+    solve, with the iteration counts of every (scale, warping)."""
+    torch = _torch()
+    q0, q1 = _pair(256, 160, 4.0, 79)
+    h, w = q0.shape
+    tv = TV_L1(w, h)
+    ns = tv.nscales
+    fs = ns - 1 if first < 0 else first
+    nx, ny = tv.scale_size(fs)
+    rng = np.random.default_rng(fs)
+    uinit = (rng.normal(size=2 * nx * ny) * 0.5).astype(np.float32)
+    d0 = torch.from_numpy(q0.reshape(-1)).cuda()
+    d1 = torch.from_numpy(q1.reshape(-1)).cuda()
+    dI = torch.from_numpy(uinit).cuda()
+    dU = torch.empty(2 * w * h, dtype=torch.float32, device="cuda")
+    tv.prepare_device_u8(d0.data_ptr(), d1.data_ptr())
+    tv.solve_prepared_schedule(dU.data_ptr(), warps, caps, dI.data_ptr(), first)
+    torch.cuda.synchronize()
+    uo, ito = O.tvl1_schedule(q0, q1, warps, caps, uinit, first=first)
+    assert np.array_equal(dU.cpu().numpy(), uo)
+    assert np.array_equal(tv.iterations(), ito)
+
+
 @pytest.mark.parametrize("first", [0, 1, 2])
 def test_solve_from_a_finer_scale_matches_oracle(first):
     """Scales coarser than `first` are skipped; the solve of the rest from a

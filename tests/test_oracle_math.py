@@ -431,3 +431,22 @@ def test_tvl1_init_none_is_plain_solve():
     a, ia = O.tvl1(q0, q1)
     b, ib = O.tvl1_init(q0, q1, None)
     assert np.array_equal(a, b) and np.array_equal(ia, ib)
+
+
+def test_tvl1_schedule_limits_each_scale():
+    """The per-scale schedule solve: uniform limits equal the plain
+    parameters' solve bit for bit; otherwise every scale keeps its own
+    warping count and iteration cap."""
+    I0, I1, _, _ = synth.make_pair(200, 150, 3.0, seed=9, bumps=True)
+    q0, q1 = synth.quantize(I0), synth.quantize(I1)
+    p = O.default_params()
+    ns = O.nscales(200, 150)
+    u, it = O.tvl1(q0, q1, p)
+    us, its = O.tvl1_schedule(q0, q1, [5] * ns, [300] * ns)
+    assert np.array_equal(u, us) and np.array_equal(it, its)
+    warps = [1, 3, 2, 5][:ns]
+    caps = [4, 300, 2, 300][:ns]
+    _, its = O.tvl1_schedule(q0, q1, warps, caps)
+    for s in range(ns):
+        assert (its[s, :warps[s]] > 0).all() and not its[s, warps[s]:].any()
+        assert its[s].max() <= caps[s]
