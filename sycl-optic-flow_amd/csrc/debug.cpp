@@ -64,7 +64,11 @@ int of_dbg_gaussian(const float* in, float* out, int w, int h, double sigma)
     float* d_tmp = B.alloc<float>(n);
     float* d_out = B.alloc<float>(n);
     OF_NN(d_in); OF_NN(d_tmp); OF_NN(d_out);
-    OF_TRY(launch_gaussian(d_in, d_tmp, d_out, w, h, g, nullptr));
+    // the tiled kernel of the solver (the same frame as both of its images),
+    // else the two-pass form
+    const int st = launch_smooth_pair(d_in, d_in, d_out, d_tmp, w, h, g, nullptr, nullptr);
+    if (st == OF_ERR_UNSUPPORTED) OF_TRY(launch_gaussian(d_in, d_tmp, d_out, w, h, g, nullptr));
+    else OF_TRY(st);
     OF_HIP(hipMemcpy(out, d_out, n * sizeof(float), hipMemcpyDeviceToHost));
     return OF_OK;
 }
@@ -85,8 +89,16 @@ int of_dbg_zoom_out(const float* in, float* out, int w, int h, float factor)
     float* d_t1 = B.alloc<float>(n);
     float* d_out = B.alloc<float>((size_t)nxx * nyy);
     OF_NN(d_in); OF_NN(d_t0); OF_NN(d_t1); OF_NN(d_out);
-    OF_TRY(launch_gaussian(d_in, d_t0, d_t1, w, h, g, nullptr));
-    OF_TRY(launch_zoom_out_sample(d_t1, d_out, w, h, nxx, nyy, factor, nullptr));
+    // as the solver's pyramid: the tiled kernel at factor 1/2
+    int st = OF_ERR_UNSUPPORTED;
+    if (factor == 0.5f)
+        st = launch_zoom_out_half_pair(d_in, d_in, d_out, d_t0, w, h, nxx, nyy, g, nullptr);
+    if (st == OF_ERR_UNSUPPORTED) {
+        OF_TRY(launch_gaussian(d_in, d_t0, d_t1, w, h, g, nullptr));
+        OF_TRY(launch_zoom_out_sample(d_t1, d_out, w, h, nxx, nyy, factor, nullptr));
+    } else {
+        OF_TRY(st);
+    }
     OF_HIP(hipMemcpy(out, d_out, (size_t)nxx * nyy * sizeof(float), hipMemcpyDeviceToHost));
     return OF_OK;
 }

@@ -670,6 +670,14 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
     }
     for (int l = 1; l < levels; l++) {
         const int pw = nx[l - 1], ph = ny[l - 1];
+        if (zf == 0.5f) {   // one tiled launch for both frames
+            const int sz = launch_zoom_out_half_pair(L0[l - 1], L1[l - 1], P0 + off[l], P1 + off[l],
+                                                     pw, ph, nx[l], ny[l], g, st);
+            if (sz != OF_ERR_UNSUPPORTED) {
+                OF_TRY(sz);
+                continue;
+            }
+        }
         const dim3 gb(cdiv(pw, 64), cdiv(ph, 4), 2), gz(cdiv(nx[l], 64), cdiv(ny[l], 4), 2);
         hipLaunchKernelGGL(k_lk_pyr_rows, gb, dim3(256), 0, st, L0[l - 1], L1[l - 1], PROD,
                            PROD + n0, pw, ph, g);

@@ -274,12 +274,28 @@ int amdflow::tvl1_prepare(of_tvl1* h, const void* src0, const void* src1, bool i
                                       h->ctl, st));
     float* I00 = h->I0p;
     float* I10 = h->I1p;
-    OF_TRY(launch_normalize(h->A0, h->A1, I00, I10, N0, h->ctl, st));
-    // pre-smoothing, tvl1.cpp:143-144
-    OF_TRY(launch_gaussian(I00, h->A0, I00, h->W, h->H, h->g_pre, st));
-    OF_TRY(launch_gaussian(I10, h->A0, I10, h->W, h->H, h->g_pre, st));
-    // pyramid, tvl1.cpp:146-154 (zoom_out: blur a copy, then resample)
+    // normalisation (tvl1.cpp:140) and pre-smoothing (:143-144) of both frames
+    // in one tiled launch; the two-pass form where the taps do not fit it
+    int st0 = launch_smooth_pair(h->A0, h->A1, I00, I10, h->W, h->H, h->g_pre, h->ctl, st);
+    if (st0 == OF_ERR_UNSUPPORTED) {
+        OF_TRY(launch_normalize(h->A0, h->A1, I00, I10, N0, h->ctl, st));
+        OF_TRY(launch_gaussian(I00, h->A0, I00, h->W, h->H, h->g_pre, st));
+        OF_TRY(launch_gaussian(I10, h->A0, I10, h->W, h->H, h->g_pre, st));
+    } else {
+        OF_TRY(st0);
+    }
+    // pyramid, tvl1.cpp:146-154 (zoom_out: blur a copy, then resample): at
+    // zfactor 1/2 one launch per scale for both frames
     for (int s = 1; s < h->ns; s++) {
+        int sz = OF_ERR_UNSUPPORTED;
+        if (P.zfactor == 0.5f)
+            sz = launch_zoom_out_half_pair(h->I0p + h->off[s - 1], h->I1p + h->off[s - 1],
+                                           h->I0p + h->off[s], h->I1p + h->off[s], h->nx[s - 1],
+                                           h->ny[s - 1], h->nx[s], h->ny[s], h->g_zoom, st);
+        if (sz != OF_ERR_UNSUPPORTED) {
+            OF_TRY(sz);
+            continue;
+        }
         for (int k = 0; k < 2; k++) {
             float* P0 = (k ? h->I1p : h->I0p);
             OF_TRY(launch_gaussian(P0 + h->off[s - 1], h->A0, h->A1, h->nx[s - 1], h->ny[s - 1],

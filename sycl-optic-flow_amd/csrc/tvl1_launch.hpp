@@ -21,6 +21,16 @@ int launch_gaussian(const float* src, float* tmp, float* dst, int nx, int ny, co
                     hipStream_t s);
 int launch_zoom_out_sample(const float* src, float* dst, int nx, int ny, int nxx, int nyy,
                            float factor, hipStream_t s);
+// Gaussian of two frames in one tiled launch (a -> oa, b -> ob, not in
+// place); with `ctl`, the frames are normalised by the keys in ctl first.
+// OF_ERR_UNSUPPORTED when the taps are wider than the tile's halo (16) or
+// the image.
+int launch_smooth_pair(const float* a, const float* b, float* oa, float* ob, int nx, int ny,
+                       const GaussTaps& g, const TvCtl* ctl, hipStream_t s);
+// zoom_out by exactly 1/2 of two frames in one launch (blur and sampling at
+// even positions); OF_ERR_UNSUPPORTED as launch_smooth_pair.
+int launch_zoom_out_half_pair(const float* a, const float* b, float* oa, float* ob, int nx, int ny,
+                              int nxx, int nyy, const GaussTaps& g, hipStream_t s);
 int launch_zoom_in_flow(const float* uc0, const float* uc1, float* uf, int nx, int ny, int nxx,
                         int nyy, float invz, const TvCtl* ctl, int coarse_level, hipStream_t s);
 int launch_zoom_in_planes(const float* a, const float* b, float* oa, float* ob, int nx, int ny,

@@ -112,6 +112,16 @@ def test_lk_pyramidal_variants_match_oracle(levels, win, iters):
     assert np.array_equal(u, ou) and np.array_equal(v, ov)
 
 
+@pytest.mark.parametrize("zf,sigma", [(0.6, 1.0), (0.5, 2.5), (0.5, 3.5)])
+def test_lk_pyramidal_zoom_and_blur_variants_match_oracle(zf, sigma):
+    """Zoom 1/2 with taps up to 16 runs the tiled pyramid kernel; another
+    zoom factor or wider taps (sigma 3.5: 18 taps) the two-pass form."""
+    I0, I1, _, _ = synth.make_pair(211, 157, 3.0, 41)
+    u, v = lk_pyramidal(I0, I1, levels=3, zf=zf, sigma=sigma, window=5, iters=3)
+    ou, ov = O.lk_pyr(I0, I1, levels=3, zf=zf, sigma=sigma, win=5, iters=3)
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+
+
 @pytest.mark.parametrize("out", ["separate", "over_I0", "over_I1"])
 def test_lk_pyramidal_device_output_placement(out):
     """The device entry point reads level 0 from the caller's frames and its

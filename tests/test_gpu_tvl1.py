@@ -61,7 +61,7 @@ def test_hypot_at_float_rounding_boundaries():
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("h,w", [(37, 53), (120, 68), (135, 240)])
+@pytest.mark.parametrize("h,w", [(37, 53), (120, 68), (135, 240), (6, 7), (1080, 1920)])
 @pytest.mark.parametrize("sigma", [float(np.float32(0.8)), float(np.float32(1.0)),
                                    float(np.float32(0.6) * np.sqrt(3.0))])
 def test_gaussian_bitexact(h, w, sigma):
@@ -70,11 +70,14 @@ def test_gaussian_bitexact(h, w, sigma):
     assert np.array_equal(D.gaussian(img, sigma), O.gaussian(img, sigma))
 
 
-@pytest.mark.parametrize("h,w", [(68, 120), (61, 97), (1080, 1920)])
-def test_zoom_out_bitexact(h, w):
+@pytest.mark.parametrize("factor", [0.5, 0.6])
+@pytest.mark.parametrize("h,w", [(68, 120), (61, 97), (1080, 1920), (7, 9), (13, 300)])
+def test_zoom_out_bitexact(h, w, factor):
+    """Factor 1/2 runs the tiled blur-and-decimate kernel, other factors the
+    two passes and the bicubic resampling."""
     rng = np.random.default_rng(7)
     img = rng_img(rng, h, w)
-    assert np.array_equal(D.zoom_out(img, 0.5), O.zoom_out(img, 0.5))
+    assert np.array_equal(D.zoom_out(img, factor), O.zoom_out(img, factor))
 
 
 @pytest.mark.parametrize("src,dst", [((120, 68), (240, 135)), ((17, 13), (33, 25)),
