@@ -126,12 +126,27 @@ int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
  * rows_per_wave in {2,4,8}; rows_per_wave 0 selects square 16 x 16 tiles,
  * one pixel per lane (iters_per_launch in {1,2,4}; the built-in choice,
  * host-paced, below 200000 pixels).  `persistent` = 1 runs each warping of those
- * scales as one cooperative launch (chunks of iters_per_launch iterations
- * separated by grid barriers) instead of host-paced launches.  Rules are
+ * scales as one launch of a co-resident grid (chunks of iters_per_launch
+ * iterations separated by grid barriers; see of_set_persist_launch_mode)
+ * instead of host-paced launches.  Rules are
  * matched largest min_pixels first; calling with min_pixels < 0 clears all
  * rules (back to the built-in table). */
 int of_tvl1_set_launch_rule(of_tvl1 *h, long long min_pixels, int iters_per_launch,
                             int rows_per_wave, int persistent);
+
+/* Launch mode of the persistent warping kernel, process-wide (no counterpart
+ * in the reference, whose solver is sequential).  OF_PERSIST_PLAIN (default):
+ * warpings whose launch rule is persistent run as one launch of a grid that
+ * fits the device at once; its first grid barrier checks within 2 ms that
+ * every workgroup started.  If one did not (another persistent kernel on
+ * another stream or process held the CUs, a CU-masked stream), the solve is
+ * repeated and the process stays in OF_PERSIST_HOST_PACED: those warpings
+ * then run host-paced with the same launch shape, which needs no
+ * co-residency.  AMDFLOW_HOST_PACED=1 in the environment starts a process in
+ * host-paced mode.  Results are the same bits in both modes. */
+enum { OF_PERSIST_PLAIN = 0, OF_PERSIST_HOST_PACED = 1 };
+int of_persist_launch_mode(void);
+int of_set_persist_launch_mode(int mode);
 
 /* Iteration launches kept queued ahead of the device within a warping
  * (1..64, default 3).  Launches queued past convergence return at once. */

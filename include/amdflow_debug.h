@@ -72,6 +72,11 @@ int of_dbg_hs_warp_derivatives(const float *I0, const float *I1, const float *u,
  * bf16-rounded operands, through the convolution kernel's MFMA path (a
  * 1x16 image of 16 channels, centre tap only). */
 int of_dbg_cnn_mfma(const float *A, const float *B, float *D);
+/* While `on` is nonzero, launches of the persistent warping kernel report at
+ * their first grid barrier that the grid was not co-resident (every
+ * workgroup arrives, then all end), so the solve takes its host-paced
+ * repeat; a test of that path. */
+int of_dbg_persist_force_not_resident(int on);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,8 @@ SIGNATURES = {
     "of_tvl1_iterations": (C.c_int, [_vp, _ip, C.c_int]),
     "of_tvl1_get_stats": (C.c_int, [_vp, C.POINTER(TVL1Stats)]),
     "of_tvl1_set_launch_rule": (C.c_int, [_vp, C.c_longlong, C.c_int, C.c_int, C.c_int]),
+    "of_persist_launch_mode": (C.c_int, []),
+    "of_set_persist_launch_mode": (C.c_int, [C.c_int]),
     "of_tvl1_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int, _ip, C.POINTER(C.c_double)]),
     "of_tvl1_set_inflight": (C.c_int, [_vp, C.c_int]),
     "of_tvl1_profile_enable": (C.c_int, [_vp, C.c_int]),
@@ -182,6 +184,21 @@ def _as(a, dtype, shape=None) -> np.ndarray:
     if shape is not None and arr.size != int(np.prod(shape)):
         raise ValueError(f"expected {int(np.prod(shape))} elements, got {arr.size}")
     return arr
+
+
+PERSIST_PLAIN, PERSIST_HOST_PACED = 0, 1
+
+
+def persist_launch_mode() -> int:
+    """Process-wide launch mode of the warpings whose launch rule is
+    persistent: PERSIST_PLAIN (one launch per warping with a residency check,
+    the default) or PERSIST_HOST_PACED (after a residency failure,
+    AMDFLOW_HOST_PACED=1, or set_persist_launch_mode)."""
+    return int(_lib.of_persist_launch_mode())
+
+
+def set_persist_launch_mode(mode: int) -> None:
+    _check(_lib.of_set_persist_launch_mode(int(mode)), "set_persist_launch_mode")
 
 
 def default_tvl1_params() -> TVL1Params:

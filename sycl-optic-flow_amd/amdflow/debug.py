@@ -19,6 +19,7 @@ _SIGS = {
     "of_dbg_tvl1_iterations": [_vp] * 9 + [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                            C.c_int, C.c_int, C.c_int, _vp],
     "of_dbg_hypot": [_vp, _vp, _vp, C.c_size_t],
+    "of_dbg_persist_force_not_resident": [C.c_int],
 }
 for _n, _a in _SIGS.items():
     _f = getattr(_lib, _n)
@@ -170,3 +171,9 @@ def tvl1_dual(I0, I1, u1, u2, params):
     _check(_lib.of_dbg_tvl1_dual(_p(I0), _p(I1), _p(a), _p(b), w, h, C.byref(params), _p(iters)),
            "dbg_tvl1_dual")
     return a, b, iters
+
+
+def persist_force_not_resident(on: bool) -> None:
+    """While on, launches of the persistent warping kernel report that their
+    grid was not co-resident, so solves take their host-paced repeat."""
+    _check(_lib.of_dbg_persist_force_not_resident(1 if on else 0), "dbg_persist_force_not_resident")

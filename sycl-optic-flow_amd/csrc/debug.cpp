@@ -252,6 +252,12 @@ int of_dbg_tvl1_iterations(const float* I1wx, const float* I1wy, const float* rh
     return OF_OK;
 }
 
+int of_dbg_persist_force_not_resident(int on)
+{
+    set_persist_force_not_resident(on != 0);
+    return OF_OK;
+}
+
 int of_dbg_hypot(const float* a, const float* b, float* out, size_t n)
 {
     if (!a || !b || !out) return OF_ERR_ARG;

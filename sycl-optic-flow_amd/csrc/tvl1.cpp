@@ -191,6 +191,10 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     a.host_status = h->hstatus;
     a.epoch = ++h->epoch;
 
+    if (persistent && persist_host_paced()) {
+        persistent = 0;   // the process gave up persistent launches
+        h->scale_P[s] = 0;
+    }
     if (persistent) {
         // the whole warping in one launch: no host involvement
         if (h->profiling) return timed_launch(h, K, R, 1, a, st);
@@ -311,12 +315,16 @@ int amdflow::tvl1_prepare(of_tvl1* h, const void* src0, const void* src1, bool i
 
 namespace {
 
+// Returned by solve_levels_once when a persistent launch found its grid not
+// fully resident: nothing was written to `dst`.
+constexpr int kRetryHostPaced = -1;
+
 // Second phase: the coarse-to-fine solve from the prepared pyramids, with
 // `init` (planar [u1|u2] at the coarsest scale, device) as the initial flow,
 // or zero when null (tvl1.cpp:157-158); at most warps_s[s] warpings and
 // max_s[s] iterations per warping at scale s.  Waits for the result.
-int solve_levels_schedule(of_tvl1* h, const float* init, int first, const int* warps_s,
-                          const int* max_s, float* dst, hipStream_t st)
+int solve_levels_once(of_tvl1* h, const float* init, int first, const int* warps_s,
+                      const int* max_s, float* dst, hipStream_t st)
 {
     const of_tvl1_params& P = h->prm;
     const size_t N0 = (size_t)h->W * h->H;
@@ -364,9 +372,12 @@ int solve_levels_schedule(of_tvl1* h, const float* init, int first, const int* w
                           hipMemcpyDeviceToHost, st));
     OF_HIP(hipStreamSynchronize(st));
     {
-        // a persistent warping whose grid barrier ran past its bound flags it
+        // a persistent warping whose grid barrier ran past its bound flags
+        // it; so does one whose grid did not fully start (the final copy
+        // then skipped `dst`)
         int derr = 0;
         OF_HIP(hipMemcpy(&derr, &h->ctl->error, sizeof(int), hipMemcpyDeviceToHost));
+        if (derr == kBarNotResident) return kRetryHostPaced;
         if (derr) {
             record_hip_error(hipErrorLaunchTimeOut, "k_tvl1_persist grid barrier", __FILE__, __LINE__);
             return OF_ERR_HIP;
@@ -383,6 +394,25 @@ int solve_levels_schedule(of_tvl1* h, const float* init, int first, const int* w
     }
     h->solved = true;
     return OF_OK;
+}
+
+// solve_levels_once, repeated once with host-paced launches (from then on,
+// for the whole process) if a persistent launch was not co-resident.  The
+// repeat starts from the same pyramids and `init`, which a solve does not
+// write.
+int solve_levels_schedule(of_tvl1* h, const float* init, int first, const int* warps_s,
+                          const int* max_s, float* dst, hipStream_t st)
+{
+    const int rc = solve_levels_once(h, init, first, warps_s, max_s, dst, st);
+    if (rc != kRetryHostPaced) return rc;
+    set_persist_host_paced(true);
+    h->timed.clear();   // launch events of the abandoned solve
+    const int rc2 = solve_levels_once(h, init, first, warps_s, max_s, dst, st);
+    if (rc2 == kRetryHostPaced) {   // the mode was set back to persistent meanwhile
+        record_hip_error(hipErrorLaunchTimeOut, "k_tvl1_persist residency", __FILE__, __LINE__);
+        return OF_ERR_HIP;
+    }
+    return rc2;
 }
 
 // The same limits at every scale.
@@ -444,10 +474,11 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
             const int P = (mode == 1 || mode == 3) ? 1 : 0;
             h->rules.assign(1, LaunchRule{0, K, R, P});
             int st = solve(h, d0, d1, true, h->dflow, hs);   // warm-up of this variant
-            // a persistent variant the device cannot run (no cooperative
-            // launch) or whose grid barrier ran past its bound is skipped;
-            // the kernels exit in both cases and the device stays usable
-            if (P && st != OF_OK && hipDeviceSynchronize() == hipSuccess) {
+            // a persistent variant whose grid barrier ran past its bound is
+            // skipped (the kernels exit and the device stays usable), and so
+            // is every persistent variant once the process runs host-paced
+            if (P && (persist_host_paced() ||
+                      (st != OF_OK && hipDeviceSynchronize() == hipSuccess))) {
                 if (table)
                     for (int s = 0; s < h->ns; s++) table[(size_t)row * h->ns + s] = -1.0;
                 continue;
@@ -1004,6 +1035,18 @@ int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_laun
         h->rules.push_back({min_pixels, iters_per_launch, rows_per_wave, p});
         return OF_OK;
     });
+}
+
+int of_persist_launch_mode(void)
+{
+    return persist_host_paced() ? OF_PERSIST_HOST_PACED : OF_PERSIST_PLAIN;
+}
+
+int of_set_persist_launch_mode(int mode)
+{
+    if (mode != OF_PERSIST_PLAIN && mode != OF_PERSIST_HOST_PACED) return OF_ERR_ARG;
+    set_persist_host_paced(mode == OF_PERSIST_HOST_PACED);
+    return OF_OK;
 }
 
 }  // extern "C"

@@ -20,7 +20,7 @@ struct LaunchRule {
     long long min_pixels;
     int K;
     int R;
-    int persistent;   // 1: whole warping in one cooperative launch
+    int persistent;   // 1: whole warping in one persistent launch
 };
 
 // Enqueues the first phase of a solve on `st`: frames to float (u8 or

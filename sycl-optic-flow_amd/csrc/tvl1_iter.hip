@@ -4,13 +4,15 @@
 //   k_tvl1_iter<K,R>     one launch = K iterations; the last workgroup to
 //                        finish evaluates the stopping rule; the host keeps a
 //                        few launches queued (large scales);
-//   k_tvl1_persist<K,R>  one cooperative launch = a whole warping; chunks of
-//                        K iterations separated by a grid barrier, every
+//   k_tvl1_persist<K,R>  one launch of a co-resident grid = a whole warping;
+//                        chunks of K iterations separated by a grid barrier, every
 //                        workgroup evaluating the stopping rule on the same
 //                        data (small, launch-latency-bound scales).
 // Arithmetic follows the reference operation by operation; the library is
 // built with -ffp-contract=off.
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 
 #include "device_math.hpp"
 #include "lockstep.hpp"
@@ -529,12 +531,30 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_iter(IterArgs a)
 // persistent driver
 // ---------------------------------------------------------------------------
 
-// Grid barrier over `nb` co-resident workgroups (cooperative launch).  Every
-// wave first waits for its own memory operations to complete, so all state
-// written by the workgroup (device-scope stores) is visible at the point of
-// coherence before the arrival.  The wait is bounded: past ~1 s it flags
-// ctl->error and returns false, and the kernel ends.
-__device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& gen, int* sh_ok)
+// Residency bound of a first grid barrier: `ticks` > 0 is the wall-clock
+// time (ticks of the device's constant-rate clock) that a workgroup waits for the others
+// before it reports kBarNotResident; `force` reports it at once (test hook).
+struct FirstBound {
+    unsigned long long ticks;
+    int force;
+};
+
+__device__ __forceinline__ bool past_first_bound(const FirstBound& fb, unsigned long long t_end,
+                                                 unsigned spins, TvCtl* ctl)
+{
+    if (!fb.ticks || (spins & 15u) != 0 || (unsigned long long)wall_clock64() < t_end) return false;
+    __hip_atomic_store(&ctl->error, kBarNotResident, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Grid barrier over `nb` co-resident workgroups.  Every wave first waits for
+// its own memory operations to complete, so all state written by the
+// workgroup (device-scope stores) is visible at the point of coherence before
+// the arrival.  The wait is bounded: past ~1 s of polls it flags kBarTimeout
+// and returns false, and the kernel ends; with a FirstBound it also ends
+// after fb.ticks of wall-clock time with kBarNotResident.
+__device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& gen, int* sh_ok,
+                                             FirstBound fb = {0ull, 0})
 {
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -542,6 +562,10 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
     if (threadIdx.x == 0) {
         int ok = 1;
         const unsigned g = gen;
+        if (fb.force)
+            __hip_atomic_store(&ctl->error, kBarNotResident, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long t_end = fb.ticks ? (unsigned long long)wall_clock64() + fb.ticks : 0ull;
         const unsigned old =
             __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == nb - 1) {
@@ -554,7 +578,12 @@ __device__ __forceinline__ bool grid_barrier(TvCtl* ctl, unsigned nb, unsigned& 
             while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
                 __builtin_amdgcn_s_sleep(2);
                 if (++spins > (1u << 21)) {
-                    __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&ctl->error, kBarTimeout, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                if (past_first_bound(fb, t_end, spins, ctl)) {
                     ok = 0;
                     break;
                 }
@@ -587,15 +616,17 @@ struct BarEpoch {
     unsigned top, grp;
 };
 
-__device__ __forceinline__ bool bar_wait(const unsigned* w, unsigned target, TvCtl* ctl)
+__device__ __forceinline__ bool bar_wait(const unsigned* w, unsigned target, TvCtl* ctl,
+                                         const FirstBound& fb, unsigned long long t_end)
 {
     unsigned spins = 0;
     while ((int)(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
         __builtin_amdgcn_s_sleep(2);
         if (++spins > (1u << 21)) {
-            __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->error, kBarTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
+        if (past_first_bound(fb, t_end, spins, ctl)) return false;
         if ((spins & 1023u) == 0 &&
             __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return false;
@@ -603,12 +634,17 @@ __device__ __forceinline__ bool bar_wait(const unsigned* w, unsigned target, TvC
     return true;
 }
 
-__device__ __forceinline__ bool grid_barrier2(TvCtl* ctl, unsigned nb, BarEpoch& ep, int* sh_ok)
+__device__ __forceinline__ bool grid_barrier2(TvCtl* ctl, unsigned nb, BarEpoch& ep, int* sh_ok,
+                                              FirstBound fb = {0ull, 0})
 {
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (fb.force)
+            __hip_atomic_store(&ctl->error, kBarNotResident, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long t_end = fb.ticks ? (unsigned long long)wall_clock64() + fb.ticks : 0ull;
         const unsigned g = blockIdx.x % kBarGroups;
         const unsigned members = nb / kBarGroups + (g < nb % kBarGroups ? 1u : 0u);
         const unsigned groups = nb < (unsigned)kBarGroups ? nb : (unsigned)kBarGroups;
@@ -626,7 +662,7 @@ __device__ __forceinline__ bool grid_barrier2(TvCtl* ctl, unsigned nb, BarEpoch&
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 __hip_atomic_store(&ctl->bar_gen, ep.top + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                ok = bar_wait(&ctl->bar_gen, ep.top + 1, ctl);
+                ok = bar_wait(&ctl->bar_gen, ep.top + 1, ctl, fb, t_end);
             }
             if (ok) {
                 __builtin_amdgcn_s_waitcnt(0);
@@ -634,10 +670,11 @@ __device__ __forceinline__ bool grid_barrier2(TvCtl* ctl, unsigned nb, BarEpoch&
                 __hip_atomic_store(gg, ep.grp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         } else {
-            ok = bar_wait(gg, ep.grp + 1, ctl);
+            ok = bar_wait(gg, ep.grp + 1, ctl, fb, t_end);
         }
         ep.top++;
         ep.grp++;
+        if (fb.force) ok = 0;
         *sh_ok = ok;
     }
     __syncthreads();
@@ -676,6 +713,8 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
     int n = 0;
     int redo = 0;
     int parity = 0;
+    // the first barrier of a plain launch also checks that the grid is resident
+    FirstBound fb{a.resident_ticks, a.force_not_resident};
     while (true) {
         int kk = redo ? redo : a.max_iter - n;
         if (kk > K) kk = K;
@@ -699,7 +738,9 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
             __hip_atomic_store(pp + (size_t)threadIdx.x * nb + blockIdx.x, part, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         // square tiles run one workgroup per small tile: many workgroups
-        if (!(SQ ? grid_barrier2(ctl, nb, ep, &sh_ok) : grid_barrier(ctl, nb, gen, &sh_ok))) return;
+        if (!(SQ ? grid_barrier2(ctl, nb, ep, &sh_ok, fb) : grid_barrier(ctl, nb, gen, &sh_ok, fb)))
+            return;
+        fb = FirstBound{0ull, 0};
         grid_sums<K, NW>(pp, nb, kk, sh4, sh_tot);
         const int keep = stop_after(sh_tot, kk, n, a);
         if (a.err_log && blockIdx.x == 0 && threadIdx.x < (unsigned)kk)
@@ -963,24 +1004,59 @@ static constexpr int persist_threads()
     return R == 0 ? 256 : 64 * kIterWaves;
 }
 
+// Launch mode of the persistent kernel.  It is launched as a plain kernel of
+// a grid that fits the device at once (occupancy x CUs), which starts every
+// workgroup when nothing else holds the CUs.  A cooperative launch would
+// guarantee co-residency, but the runtime brackets it with about 11 us of
+// idle device time on each side (kernel trace of the 1920x1080 solve, 20
+// such launches per solve: 6.06 vs 5.69 ms per solve).  Instead the first
+// grid barrier of each launch checks residency with a wall-clock bound: if
+// the grid did not fully start (another persistent kernel on another stream
+// or process holding CUs, a CU-masked stream), the kernel ends with
+// kBarNotResident, the host switches the process to host-paced launches for
+// the warpings that would be persistent, and repeats the solve.  Host-paced
+// launches need no co-residency and give the same bits.
+static std::atomic<int> g_host_paced{-1};   // -1: not read from the environment yet
+static std::atomic<int> g_force_not_resident{0};
+constexpr double kResidentBoundUs = 2000.0;   // first-barrier wait of a launch
+
+bool persist_host_paced()
+{
+    int m = g_host_paced.load(std::memory_order_relaxed);
+    if (m < 0) {
+        const char* e = std::getenv("AMDFLOW_HOST_PACED");
+        const int want = (e && e[0] == '1') ? 1 : 0;
+        g_host_paced.compare_exchange_strong(m, want);
+        m = g_host_paced.load(std::memory_order_relaxed);
+    }
+    return m == 1;
+}
+
+void set_persist_host_paced(bool on) { g_host_paced.store(on ? 1 : 0); }
+
+void set_persist_force_not_resident(bool on) { g_force_not_resident.store(on ? 1 : 0); }
+
 template <int K, int R>
 static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
 {
     static int resident = -1;   // co-resident workgroups of this kernel on the device
+    static unsigned long long bound_ticks = 0;
     if (resident < 0) {
-        int dev = 0, per_cu = 0, cus = 0, coop = 0;
+        int dev = 0, per_cu = 0, cus = 0, clock_khz = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel<K, R>(),
                                                          persist_threads<R>(), 0) != hipSuccess) {
             (void)hipGetLastError();
             return OF_ERR_HIP;
         }
+        // the residency bound in wall-clock ticks (at least 1 ms at any rate)
+        bound_ticks = (unsigned long long)(kResidentBoundUs * 1e-3 * std::max(clock_khz, 1000));
         // at most 4 workgroups per CU: 1024 arrivals on one counter and
         // pollers of one word per barrier are already plenty (the compiled
         // bound allows 2 of 8 waves)
-        resident = coop ? std::min(per_cu, 4) * cus : 0;
+        resident = std::min(per_cu, 4) * cus;
     }
     if (resident <= 0) return OF_ERR_UNSUPPORTED;
     unsigned gx, gy;
@@ -990,12 +1066,14 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
     if (nb > ntiles) nb = ntiles;
     if (nb > (unsigned)resident) nb = (unsigned)resident;
     IterArgs args = a;
+    args.resident_ticks = bound_ticks;
+    args.force_not_resident = g_force_not_resident.load(std::memory_order_relaxed);
     unsigned tx = gx, nt = ntiles;
     void* kargs[] = {(void*)&args, (void*)&tx, (void*)&nt};
-    const hipError_t e = hipLaunchCooperativeKernel(persist_kernel<K, R>(), dim3(nb),
-                                                    dim3(persist_threads<R>()), kargs, 0, s);
+    const hipError_t e =
+        hipLaunchKernel(persist_kernel<K, R>(), dim3(nb), dim3(persist_threads<R>()), kargs, 0, s);
     if (e != hipSuccess) {
-        record_hip_error(e, "k_tvl1_persist cooperative launch", __FILE__, __LINE__);
+        record_hip_error(e, "k_tvl1_persist launch", __FILE__, __LINE__);
         return OF_ERR_HIP;
     }
     return OF_OK;

@@ -415,6 +415,7 @@ __global__ __launch_bounds__(256) void k_copy_flow(const float* __restrict__ ub0
                                                    float* __restrict__ out, unsigned n2,
                                                    const TvCtl* ctl, int level)
 {
+    if (ctl->error) return;   // a failed solve leaves the destination as it was
     const float* ub = ctl->cur[level] ? ub1 : ub0;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x)
         out[i] = ub[i];

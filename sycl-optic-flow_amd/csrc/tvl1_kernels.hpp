@@ -38,8 +38,16 @@ struct TvCtl {
     alignas(256) unsigned bar_gen;
     BarWord grp_count[kBarGroups];   // two-level barrier: per-group arrivals
     BarWord grp_gen[kBarGroups];     // and per-group release epochs
-    alignas(256) int error;         // set if a barrier wait ran past its bound
+    alignas(256) int error;         // kBarTimeout or kBarNotResident, else 0
 };
+
+// ctl->error values.  kBarTimeout: a grid barrier wait ran past its poll
+// bound.  kBarNotResident: the first grid barrier of a persistent launch did
+// not see every workgroup arrive within its wall-clock bound, so the grid may
+// not be co-resident; the host then repeats the solve with host-paced
+// launches.
+constexpr int kBarTimeout = 1;
+constexpr int kBarNotResident = 2;
 
 struct IterArgs {
     const float* wx;
@@ -59,15 +67,27 @@ struct IterArgs {
     unsigned long long* host_status;  // host-mapped progress word (may be null)
     unsigned epoch;
     double* err_log;    // if set: per-iteration error sums, indexed by iteration
+    // persistent kernel: wall-clock ticks that the first grid barrier waits
+    // for all workgroups before it reports kBarNotResident (0: no bound)
+    unsigned long long resident_ticks;
+    int force_not_resident;   // test hook: the first barrier reports kBarNotResident
 };
 
 // Launch helpers implemented in tvl1_iter.hip.
 int launch_iter(int K, int R, const IterArgs& a, hipStream_t s);
 bool iter_config_ok(int K, int R);
 void iter_grid(int K, int R, int nx, int ny, unsigned* gx, unsigned* gy);
-// Whole warping in one cooperative launch; `nblocks` = 0 picks the largest
-// co-resident grid not above the tile count.  Returns OF_ERR_UNSUPPORTED if
-// the device cannot launch cooperatively.
+// Whole warping in one launch of a grid that fits the device at once;
+// `nblocks` = 0 picks the largest such grid not above the tile count.  Its
+// first grid barrier reports kBarNotResident if the grid did not fully start.
 int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s);
+// Process-wide: true = warpings whose launch rule is persistent run
+// host-paced instead (after a residency failure, with AMDFLOW_HOST_PACED=1,
+// or when set); false (default) = persistent launches.
+bool persist_host_paced();
+void set_persist_host_paced(bool on);
+// Test hook: persistent launches report kBarNotResident at their first
+// barrier while set.
+void set_persist_force_not_resident(bool on);
 
 }  // namespace amdflow
