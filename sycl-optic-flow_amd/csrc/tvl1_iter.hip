@@ -1009,7 +1009,7 @@ static constexpr int persist_threads()
 // workgroup when nothing else holds the CUs.  A cooperative launch would
 // guarantee co-residency, but the runtime brackets it with about 11 us of
 // idle device time on each side (kernel trace of the 1920x1080 solve, 20
-// such launches per solve: 6.06 vs 5.69 ms per solve).  Instead the first
+// such launches per solve: 6.07 vs 5.67 ms per solve).  Instead the first
 // grid barrier of each launch checks residency with a wall-clock bound: if
 // the grid did not fully start (another persistent kernel on another stream
 // or process holding CUs, a CU-masked stream), the kernel ends with

@@ -25,3 +25,24 @@ def test_help_lists_the_profile_flags():
     for flag in ("--steps", "--warmup", "--profile-steps", "--no-cpu", "--batch-pairs",
                  "--warm-pairs", "--no-other", "--copy-too", "--batch-reps", "--no-batch-workers"):
         assert flag in r.stdout, flag
+
+
+def _bench_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_traffic_hash_ignores_comments_not_code():
+    """The measured HBM traffic is reported only for the kernel code it was
+    measured on: comments and layout do not count, code and strings do."""
+    b = _bench_module()
+    base = 'int f(int a) { return a * 2; }  // twice\n/* note */ const char* s = "a // b";\n'
+    same = '/* header */\nint f(int a)\n{\n    return a * 2;   // doubled\n}\nconst char* s = "a // b";'
+    assert b._code_only(base) == b._code_only(same)
+    assert b._code_only(base) != b._code_only(base.replace("a * 2", "a * 3"))
+    assert b._code_only(base) != b._code_only(base.replace('"a // b"', '"a b"'))
+    assert all(os.path.exists(os.path.join(ROOT, f)) for f in b.TRAFFIC_SOURCES)
+    assert len(b._sources_hash()) == 16
