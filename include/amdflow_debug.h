@@ -77,6 +77,15 @@ int of_dbg_cnn_mfma(const float *A, const float *B, float *D);
  * workgroup arrives, then all end), so the solve takes its host-paced
  * repeat; a test of that path. */
 int of_dbg_persist_force_not_resident(int on);
+/* Allocation failure injection: after this call the first `n` device or
+ * pinned host allocations of the library succeed and every later one fails
+ * with OF_ERR_ALLOC, as a real failure would (n < 0: none fails).  The call
+ * also restarts the attempt count of of_dbg_alloc_attempts. */
+int of_dbg_fail_alloc_after(long long n);
+/* Allocation attempts since the last of_dbg_fail_alloc_after. */
+long long of_dbg_alloc_attempts(void);
+/* Device and pinned host blocks the library holds now. */
+long long of_dbg_live_allocs(void);
 
 #ifdef __cplusplus
 }

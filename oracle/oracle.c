@@ -38,6 +38,7 @@
  * restatements below implement the algorithms' stated intent with the
  * border rules written next to each function.
  */
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -181,15 +182,32 @@ static inline float cubic_cell(const float v[4], float x)
     return (float)((double)v[1] + (0.5 * xd) * inner);
 }
 
+/* float -> int truncation, defined for every input: NaN gives 0, values
+ * beyond the int range saturate (C leaves both undefined; the GPU's
+ * conversion instruction behaves this way). */
+static inline int f2i_sat(double x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return INT_MAX;
+    if (x <= -2147483648.0) return INT_MIN;
+    return (int)x;
+}
+
+/* A tap base index limited to [-4, n+4]: every tap within 2 of it then
+ * clamps to the same pixel and sets `out` as without the limit, and the tap
+ * offsets cannot overflow. */
+static inline int tap_base(int i, int n) { return i < -4 ? -4 : (i > n + 4 ? n + 4 : i); }
+
 /* bicubic_interpolation.cpp:74-133.  Tap columns (int)uu-sx .. (int)uu+2sx,
  * tap rows (int)vv-sx .. (int)vv+2sy: the row above uses the x sign, as in
- * the reference (:92); any clamped tap sets `out`. */
+ * the reference (:92); any clamped tap sets `out`.  Non-finite coordinates
+ * take the conversion of f2i_sat. */
 float orc_bicubic_at(const float *input, float uu, float vv, int nx, int ny, int border_out)
 {
     const int sx = (uu < 0) ? -1 : 1;
     const int sy = (vv < 0) ? -1 : 1;
     int out = 0;
-    const int iu = (int)uu, iv = (int)vv;
+    const int iu = tap_base(f2i_sat(uu), nx), iv = tap_base(f2i_sat(vv), ny);
     const int x = neumann_clamp(iu, nx, &out);
     const int y = neumann_clamp(iv, ny, &out);
     const int mx = neumann_clamp(iu - sx, nx, &out);
@@ -765,7 +783,8 @@ float orc_bilinear_at(const float *src, int w, int h, int stride, float x, float
 {
     const float fx0 = floorf(x), fy0 = floorf(y);
     const float ax = x - fx0, ay = y - fy0;
-    const int x0 = (int)fx0, y0 = (int)fy0;
+    /* limited to [-1, w] first so that +1 cannot overflow (clamps unchanged) */
+    const int x0 = clampi(f2i_sat(fx0), -1, w), y0 = clampi(f2i_sat(fy0), -1, h);
     const int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
     const int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
     const float a = src[(size_t)ya * stride + xa], b = src[(size_t)ya * stride + xb];
@@ -884,7 +903,13 @@ int orc_hs_compute(const float *I0, const float *I1, int width, int height, int 
     for (; L > 0; --L) {
         const int nw = pW[L] / 2, nh = pH[L] / 2;
         const int ns = ((nw + 31) / 32) * 32;
-        if (nw < 1 || nh < 1) return ORC_ERR_ARG;
+        if (nw < 1 || nh < 1) {   /* too many levels for the frame */
+            for (int k = L; k < nLevels; k++) {
+                free(pI0[k]);
+                free(pI1[k]);
+            }
+            return ORC_ERR_ARG;
+        }
         pI0[L - 1] = (float *)calloc((size_t)ns * nh, sizeof(float));
         pI1[L - 1] = (float *)calloc((size_t)ns * nh, sizeof(float));
         orc_hs_downscale(pI0[L], pW[L], pH[L], pS[L], pI0[L - 1], nw, nh, ns);
@@ -1084,7 +1109,12 @@ int orc_lk_pyr_compute(const float *I0, const float *I1, int width, int height, 
         const size_t nl = (size_t)nx[l] * ny[l];
         P0[l] = (float *)malloc(sizeof(float) * nl);
         P1[l] = (float *)malloc(sizeof(float) * nl);
-        /* blur a copy, then sample at (j/zf, i/zf) as zoom.cpp:28-53 does */
+        /* blur a copy, then sample at (j/zf, i/zf) as zoom.cpp:28-53 does.
+         * This pyramid has no reference counterpart (the reference LK is
+         * single-scale); at zf = 1/2 its sample is defined as the blurred
+         * pixel (2j, 2i) itself, which the bicubic at those integer positions
+         * also returns except for IEEE effects of its zero-weight terms (a
+         * NaN or infinite neighbour, the sign of a zero). */
         for (int k = 0; k < 2 && !st; k++) {
             const float *src = k ? P1[l - 1] : P0[l - 1];
             float *dst = k ? P1[l] : P0[l];
@@ -1093,11 +1123,20 @@ int orc_lk_pyr_compute(const float *I0, const float *I1, int width, int height, 
             if (st) break;
             for (int i1 = 0; i1 < ny[l]; i1++)
                 for (int j1 = 0; j1 < nx[l]; j1++)
-                    dst[(size_t)i1 * nx[l] + j1] = orc_bicubic_at(scratch, (float)j1 / zf, (float)i1 / zf,
+                    dst[(size_t)i1 * nx[l] + j1] =
+                        zf == 0.5f ? scratch[(size_t)(2 * i1) * nx[l - 1] + 2 * j1]
+                                   : orc_bicubic_at(scratch, (float)j1 / zf, (float)i1 / zf,
                                                                   nx[l - 1], ny[l - 1], 0);
         }
+        if (st) {   /* levels 0..l hold buffers */
+            for (int k = 0; k <= l; k++) {
+                free(P0[k]);
+                free(P1[k]);
+            }
+            free(scratch);
+            return st;
+        }
     }
-    if (st) return st;
     const int wc = (win - 1) / 2;
     float *cu = (float *)calloc(n0, sizeof(float)), *cv = (float *)calloc(n0, sizeof(float));
     float *nu = (float *)calloc(n0, sizeof(float)), *nv = (float *)calloc(n0, sizeof(float));

@@ -24,6 +24,7 @@
  * are widened to double (epsilon^2 is formed in double).  Frames are float
  * inputs and are read exactly.
  */
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -115,6 +116,18 @@ static inline int neumann(int x, int n, int *out)
     return x;
 }
 
+/* float -> int truncation defined for every input (NaN 0, saturating) and
+ * the tap base limited to [-4, n+4], as in oracle.c. */
+static inline int f2i_sat(double x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return INT_MAX;
+    if (x <= -2147483648.0) return INT_MIN;
+    return (int)x;
+}
+
+static inline int tap_base(int i, int n) { return i < -4 ? -4 : (i > n + 4 ? n + 4 : i); }
+
 /* bicubic_interpolation.cpp:74-133: taps at (int)uu - s .. (int)uu + 2s in x
  * and (int)vv - s .. +2s in y (s the sign of the coordinate; the row above
  * takes the x sign), clamped; with border_out, 0 when any tap was clamped. */
@@ -122,7 +135,7 @@ double o64_bicubic_at(const double *I, double uu, double vv, int nx, int ny, int
 {
     const int sx = uu < 0 ? -1 : 1, sy = vv < 0 ? -1 : 1;
     int out = 0;
-    const int iu = (int)uu, iv = (int)vv;
+    const int iu = tap_base(f2i_sat(uu), nx), iv = tap_base(f2i_sat(vv), ny);
     const int x = neumann(iu, nx, &out), y = neumann(iv, ny, &out);
     const int mx = neumann(iu - sx, nx, &out), my = neumann(iv - sx, ny, &out);
     const int dx = neumann(iu + sx, nx, &out), dy = neumann(iv + sy, ny, &out);
@@ -387,7 +400,7 @@ int o64_tvl1_run_f32(const float *I0in, const float *I1in, int width, int height
 static double bilinear64(const double *src, int w, int h, double x, double y)
 {
     const double fx = floor(x), fy = floor(y), ax = x - fx, ay = y - fy;
-    const int x0 = (int)fx, y0 = (int)fy;
+    const int x0 = clampi(f2i_sat(fx), -1, w), y0 = clampi(f2i_sat(fy), -1, h);
     const int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
     const int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
     const double a = src[(size_t)ya * w + xa], b = src[(size_t)ya * w + xb];
@@ -465,7 +478,13 @@ int o64_hs_compute(const float *I0, const float *I1, int width, int height, int 
         }
     for (; L > 0; --L) {
         const int nw = pW[L] / 2, nh = pH[L] / 2;
-        if (nw < 1 || nh < 1) return ORC_ERR_ARG;
+        if (nw < 1 || nh < 1) {   /* too many levels for the frame */
+            for (int k = L; k < nLevels; k++) {
+                free(pI0[k]);
+                free(pI1[k]);
+            }
+            return ORC_ERR_ARG;
+        }
         pI0[L - 1] = (double *)malloc(sizeof(double) * (size_t)nw * nh);
         pI1[L - 1] = (double *)malloc(sizeof(double) * (size_t)nw * nh);
         o64_hs_downscale(pI0[L], pW[L], pH[L], pI0[L - 1], nw, nh);
@@ -586,11 +605,19 @@ int o64_lk_pyr_compute(const float *I0, const float *I1, int width, int height, 
             double *dst = k ? P1[l] : P0[l];
             for (int i = 0; i < ny[l]; i++)
                 for (int j = 0; j < nx[l]; j++)
-                    dst[(size_t)i * nx[l] + j] = o64_bicubic_at(blur, j / zf, i / zf, nx[l - 1], ny[l - 1], 0);
+                    dst[(size_t)i * nx[l] + j] =   /* zf = 1/2: the blurred pixel, as oracle.c */
+                        zf == 0.5 ? blur[(size_t)(2 * i) * nx[l - 1] + 2 * j]
+                                  : o64_bicubic_at(blur, j / zf, i / zf, nx[l - 1], ny[l - 1], 0);
         }
         free(blur);
+        if (st) {   /* levels 0..l hold buffers */
+            for (int k = 0; k <= l; k++) {
+                free(P0[k]);
+                free(P1[k]);
+            }
+            return st;
+        }
     }
-    if (st) return st;
     const int wc = (win - 1) / 2;
     double *m = (double *)calloc(13 * n0, sizeof(double));
     if (!m) return ORC_ERR_ALLOC;

@@ -20,6 +20,7 @@ _SIGS = {
                                            C.c_int, C.c_int, C.c_int, _vp],
     "of_dbg_hypot": [_vp, _vp, _vp, C.c_size_t],
     "of_dbg_persist_force_not_resident": [C.c_int],
+    "of_dbg_fail_alloc_after": [C.c_longlong],
 }
 for _n, _a in _SIGS.items():
     _f = getattr(_lib, _n)
@@ -177,3 +178,24 @@ def persist_force_not_resident(on: bool) -> None:
     """While on, launches of the persistent warping kernel report that their
     grid was not co-resident, so solves take their host-paced repeat."""
     _check(_lib.of_dbg_persist_force_not_resident(1 if on else 0), "dbg_persist_force_not_resident")
+
+
+_lib.of_dbg_alloc_attempts.restype = C.c_longlong
+_lib.of_dbg_alloc_attempts.argtypes = []
+_lib.of_dbg_live_allocs.restype = C.c_longlong
+_lib.of_dbg_live_allocs.argtypes = []
+
+
+def fail_alloc_after(n: int) -> None:
+    """The first n device or pinned host allocations from now succeed, every
+    later one fails with OF_ERR_ALLOC (n < 0: none fails); restarts the
+    attempt count."""
+    _check(_lib.of_dbg_fail_alloc_after(int(n)), "dbg_fail_alloc_after")
+
+
+def alloc_attempts() -> int:
+    return int(_lib.of_dbg_alloc_attempts())
+
+
+def live_allocs() -> int:
+    return int(_lib.of_dbg_live_allocs())

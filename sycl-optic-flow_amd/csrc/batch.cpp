@@ -202,11 +202,11 @@ void ls_release(of_tvl1_batch* b)
         amdflow::dev_free(L.d_pairs);
         amdflow::dev_free(L.d_bases);
         amdflow::dev_free(L.d_iters);
-        if (L.hstatus) (void)hipHostFree(L.hstatus);
-        if (L.h_preps) (void)hipHostFree(L.h_preps);
-        if (L.h_pairs) (void)hipHostFree(L.h_pairs);
-        if (L.h_bases) (void)hipHostFree(L.h_bases);
-        if (L.h_iters) (void)hipHostFree(L.h_iters);
+        amdflow::host_free(L.hstatus);
+        amdflow::host_free(L.h_preps);
+        amdflow::host_free(L.h_pairs);
+        amdflow::host_free(L.h_bases);
+        amdflow::host_free(L.h_iters);
     }
     b->lanes.clear();
 }
@@ -218,16 +218,10 @@ int ls_reserve(T** host, T** dev, size_t* cap, size_t n)
 {
     if (n <= *cap) return OF_OK;
     amdflow::dev_free(*dev);
-    if (*host) (void)hipHostFree(*host);
-    *dev = nullptr;
-    *host = nullptr;
+    amdflow::host_free(*host);
     *cap = 0;
     const size_t want = std::max(n, (size_t)64);
-    if (hipHostMalloc((void**)host, sizeof(T) * want, hipHostMallocDefault) != hipSuccess) {
-        *host = nullptr;
-        (void)hipGetLastError();
-        return OF_ERR_ALLOC;
-    }
+    OF_TRY(amdflow::host_alloc(host, want, hipHostMallocDefault));
     OF_TRY(amdflow::dev_alloc(dev, want));
     *cap = want;
     return OF_OK;
@@ -238,12 +232,7 @@ int ls_setup(LsLane& L)
     if (L.mc) return OF_OK;
     OF_TRY(amdflow::dev_alloc(&L.mc, 1));
     OF_HIP(hipMemset(L.mc, 0, sizeof(amdflow::LsCtl)));
-    if (hipHostMalloc((void**)&L.hstatus, sizeof(unsigned long long),
-                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-        L.hstatus = nullptr;
-        (void)hipGetLastError();
-        return OF_ERR_ALLOC;
-    }
+    OF_TRY(amdflow::host_alloc(&L.hstatus, 1, hipHostMallocMapped | hipHostMallocCoherent));
     *L.hstatus = 0ull;
     return OF_OK;
 }

@@ -1,11 +1,27 @@
 // common.cpp -- status strings, HIP error recording, Gaussian taps.
 #include "common.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 
+#include "amdflow_debug.h"
+
 namespace amdflow {
+
+static std::atomic<long long> g_fail_after{-1};   // < 0: no injected failure
+static std::atomic<long long> g_attempts{0};      // since the last of_dbg_fail_alloc_after
+static std::atomic<long long> g_live{0};
+
+bool alloc_attempt_fails()
+{
+    const long long k = g_attempts.fetch_add(1, std::memory_order_relaxed);
+    const long long n = g_fail_after.load(std::memory_order_relaxed);
+    return n >= 0 && k >= n;
+}
+
+void alloc_live_add(long long d) { g_live.fetch_add(d, std::memory_order_relaxed); }
 
 static thread_local char g_last_error[512];
 
@@ -57,3 +73,18 @@ extern "C" const char* of_build_info(void)
 {
     return "amdflow gfx950 hip; -O3 -ffp-contract=off, IEEE fp32 div/sqrt, no packed fp32";
 }
+
+extern "C" {
+
+int of_dbg_fail_alloc_after(long long n)
+{
+    amdflow::g_attempts.store(0);
+    amdflow::g_fail_after.store(n < 0 ? -1 : n);
+    return OF_OK;
+}
+
+long long of_dbg_alloc_attempts(void) { return amdflow::g_attempts.load(); }
+
+long long of_dbg_live_allocs(void) { return amdflow::g_live.load(); }
+
+}  // extern "C"

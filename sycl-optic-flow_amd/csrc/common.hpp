@@ -43,24 +43,62 @@ int guarded(F&& body) noexcept
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Allocation accounting behind dev_alloc/host_alloc: every attempt is
+// counted, and while a failure point is set (of_dbg_fail_alloc_after) the
+// attempts from that one on fail as a real allocation failure would; live
+// blocks are counted so that tests can see a leak on any error path.
+bool alloc_attempt_fails();
+void alloc_live_add(long long d);
+
 // Device allocation that reports failure as a status code.
 template <typename T>
 int dev_alloc(T** p, size_t count)
 {
     *p = nullptr;
     if (count == 0) count = 1;
+    if (alloc_attempt_fails()) return OF_ERR_ALLOC;
     if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return OF_ERR_ALLOC;
     }
+    alloc_live_add(1);
     return OF_OK;
 }
 
 template <typename T>
 void dev_free(T*& p)
 {
-    if (p) (void)hipFree((void*)p);
+    if (p) {
+        (void)hipFree((void*)p);
+        alloc_live_add(-1);
+    }
+    p = nullptr;
+}
+
+// Pinned host allocation (hipHostMalloc flags), accounted as dev_alloc.
+template <typename T>
+int host_alloc(T** p, size_t count, unsigned flags)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (alloc_attempt_fails()) return OF_ERR_ALLOC;
+    if (hipHostMalloc((void**)p, count * sizeof(T), flags) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return OF_ERR_ALLOC;
+    }
+    alloc_live_add(1);
+    return OF_OK;
+}
+
+template <typename T>
+void host_free(T*& p)
+{
+    if (p) {
+        (void)hipHostFree((void*)p);
+        alloc_live_add(-1);
+    }
     p = nullptr;
 }
 

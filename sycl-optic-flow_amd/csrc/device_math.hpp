@@ -4,6 +4,7 @@
 // -ffp-contract=off (no fused multiply-add) and IEEE fp32 division/sqrt, the
 // GPU produces the same bits as the CPU.
 #pragma once
+#include <climits>
 #include <hip/hip_runtime.h>
 
 namespace amdflow {
@@ -41,13 +42,26 @@ struct BicubicTaps {
     bool out;
 };
 
+// float -> int truncation defined for every input: NaN gives 0, values
+// beyond the int range saturate (the conversion instruction's own results,
+// spelled out because the language leaves them undefined).
+__device__ __forceinline__ int f2i_sat(float x)
+{
+    return x != x ? 0 : (x >= 2147483647.0f ? INT_MAX : (x <= -2147483648.0f ? INT_MIN : (int)x));
+}
+
+// A tap base index limited to [-4, n+4]: every tap within 2 of it clamps to
+// the same pixel and sets `out` as without the limit, and the offsets
+// cannot overflow.
+__device__ __forceinline__ int tap_base(int i, int n) { return i < -4 ? -4 : (i > n + 4 ? n + 4 : i); }
+
 __device__ __forceinline__ BicubicTaps bicubic_taps(float uu, float vv, int nx, int ny)
 {
     BicubicTaps t;
     const int sx = (uu < 0) ? -1 : 1;
     const int sy = (vv < 0) ? -1 : 1;
     bool out = false;
-    const int iu = (int)uu, iv = (int)vv;
+    const int iu = tap_base(f2i_sat(uu), nx), iv = tap_base(f2i_sat(vv), ny);
     const int x = neumann_clamp(iu, nx, out);
     const int y = neumann_clamp(iv, ny, out);
     const int mx = neumann_clamp(iu - sx, nx, out);

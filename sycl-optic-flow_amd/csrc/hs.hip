@@ -28,7 +28,8 @@ __device__ __forceinline__ float bilinear_at(const float* __restrict__ src, int 
 {
     const float fx0 = floorf(x), fy0 = floorf(y);
     const float ax = x - fx0, ay = y - fy0;
-    const int x0 = (int)fx0, y0 = (int)fy0;
+    // limited to [-1, w] first so that +1 cannot overflow (clamps unchanged)
+    const int x0 = clampi(f2i_sat(fx0), -1, w), y0 = clampi(f2i_sat(fy0), -1, h);
     const int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
     const int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
     const float a = src[(size_t)ya * s + xa], b = src[(size_t)ya * s + xb];
@@ -400,8 +401,9 @@ int hs_solve(of_hs* h, const float* dI0, const float* dI1, int width, int height
             t = h->v; h->v = h->nv; h->nv = t;
         }
     }
-    OF_HIP(hipMemcpyAsync(du_out, h->u, fsize * sizeof(float), hipMemcpyDeviceToDevice, st));
-    OF_HIP(hipMemcpyAsync(dv_out, h->v, fsize * sizeof(float), hipMemcpyDeviceToDevice, st));
+    // the result is in h->u/h->v, which the level loop swaps with nu/nv
+    if (du_out) OF_HIP(hipMemcpyAsync(du_out, h->u, fsize * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (dv_out) OF_HIP(hipMemcpyAsync(dv_out, h->v, fsize * sizeof(float), hipMemcpyDeviceToDevice, st));
     OF_HIP(hipStreamSynchronize(st));
     return OF_OK;
 }
@@ -495,10 +497,12 @@ int of_hs_compute(of_hs* h, const float* I0, const float* I1, int width, int hei
     OF_HIP(hipMemcpyAsync(h->hin0, I0, fsize * sizeof(float), hipMemcpyHostToDevice, h->stream));
     OF_HIP(hipMemcpyAsync(h->hin1, I1, fsize * sizeof(float), hipMemcpyHostToDevice, h->stream));
     OF_TRY(hs_solve(h, h->hin0, h->hin1, width, height, stride, alpha, nLevels, nWarpIters,
-                    nSolverIters, h->nu, h->nv, h->stream));
-    // hs_solve leaves the result in h->u/h->v and copies it into nu/nv
-    OF_HIP(hipMemcpyAsync(u, h->nu, fsize * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-    OF_HIP(hipMemcpyAsync(v, h->nv, fsize * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+                    nSolverIters, nullptr, nullptr, h->stream));
+    // read the buffers after the solve: every level but the last swaps u
+    // with nu (and v with nv), so which one holds the flow depends on the
+    // level count
+    OF_HIP(hipMemcpyAsync(u, h->u, fsize * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    OF_HIP(hipMemcpyAsync(v, h->v, fsize * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     OF_HIP(hipStreamSynchronize(h->stream));
     return OF_OK;
 }

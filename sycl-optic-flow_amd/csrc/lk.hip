@@ -23,6 +23,7 @@
 #include "amdflow.h"
 #include "amdflow_debug.h"
 #include "common.hpp"
+#include "device_math.hpp"
 #include "tvl1_launch.hpp"
 #include "tvl1_pixel.hpp"
 
@@ -143,7 +144,8 @@ __device__ __forceinline__ float bilinear_at(const float* __restrict__ src, int 
 {
     const float fx0 = floorf(x), fy0 = floorf(y);
     const float ax = x - fx0, ay = y - fy0;
-    const int x0 = (int)fx0, y0 = (int)fy0;
+    // limited to [-1, w] first so that +1 cannot overflow (clamps unchanged)
+    const int x0 = clampi(f2i_sat(fx0), -1, w), y0 = clampi(f2i_sat(fy0), -1, h);
     const int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
     const int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
     const float a = src[(size_t)ya * w + xa], b = src[(size_t)ya * w + xb];
@@ -430,7 +432,7 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
         const float fx0 = floorf(x), fy0 = floorf(y);
         fx[k] = x - fx0;
         fy[k] = y - fy0;
-        const int xi = (int)fx0, yi = (int)fy0;
+        const int xi = clampi(f2i_sat(fx0), -1, w), yi = clampi(f2i_sat(fy0), -1, h);
         const int xa = clampi(xi, 0, w - 1), xb = clampi(xi + 1, 0, w - 1);
         const int ya = clampi(yi, 0, h - 1), yb = clampi(yi + 1, 0, h - 1);
         ta[k] = P1[(size_t)ya * w + xa];

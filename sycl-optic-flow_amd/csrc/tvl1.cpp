@@ -93,8 +93,8 @@ void release(of_tvl1* h)
     dev_free(h->partials);
     dev_free(h->energy);
     dev_free(h->d_iters);
-    if (h->hflow) (void)hipHostFree(h->hflow);
-    if (h->hstatus) (void)hipHostFree(h->hstatus);
+    host_free(h->hflow);
+    host_free(h->hstatus);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : h->scale_ev) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -289,10 +289,15 @@ int amdflow::tvl1_prepare(of_tvl1* h, const void* src0, const void* src1, bool i
         OF_TRY(st0);
     }
     // pyramid, tvl1.cpp:146-154 (zoom_out: blur a copy, then resample): at
-    // zfactor 1/2 one launch per scale for both frames
+    // zfactor 1/2 one launch per scale for both frames, which keeps only the
+    // blurred samples at even positions.  That equals the bicubic resampling
+    // of every finite frame without negative zeros, as u8 frames normalise
+    // to (finite values >= +0); there bicubic_at(2j, 2i) adds (0.5*0)*inner
+    // to the sample, and a non-finite neighbour turns that into NaN, a -0
+    // sample may turn into +0.  Float frames therefore take the two-step path.
     for (int s = 1; s < h->ns; s++) {
         int sz = OF_ERR_UNSUPPORTED;
-        if (P.zfactor == 0.5f)
+        if (P.zfactor == 0.5f && is_u8)
             sz = launch_zoom_out_half_pair(h->I0p + h->off[s - 1], h->I1p + h->off[s - 1],
                                            h->I0p + h->off[s], h->I1p + h->off[s], h->nx[s - 1],
                                            h->ny[s - 1], h->nx[s], h->ny[s], h->g_zoom, st);
@@ -618,16 +623,9 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         if (!st) st = dev_alloc(&h->partials, 2 * maxpart);
         if (!st) st = dev_alloc(&h->energy, 2 * kEnergyBlocks + 3);
         if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
-        if (!st && hipHostMalloc((void**)&h->hflow, sizeof(float) * 2 * N0, hipHostMallocDefault) != hipSuccess) {
-            h->hflow = nullptr;
-            st = OF_ERR_ALLOC;
-        }
-        if (!st && hipHostMalloc((void**)&h->hstatus, sizeof(unsigned long long),
-                                 hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-            h->hstatus = nullptr;
-            st = OF_ERR_ALLOC;
-        }
-        if (st) { (void)hipGetLastError(); release(h); return st; }
+        if (!st) st = host_alloc(&h->hflow, 2 * N0, hipHostMallocDefault);
+        if (!st) st = host_alloc(&h->hstatus, 1, hipHostMallocMapped | hipHostMallocCoherent);
+        if (st) { release(h); return st; }
         *h->hstatus = 0ull;
         std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
         h->h_iters.assign((size_t)ns * p.warps, 0);

@@ -103,10 +103,17 @@ __device__ __forceinline__ void minmax_keys(float va, float vb, unsigned i, unsi
     const unsigned long long ka = (unsigned long long)__float_as_uint(fabsf(va)) << 32;
     const unsigned long long kb = (unsigned long long)__float_as_uint(fabsf(vb)) << 32;
     const unsigned long long hi = 0xFFFFFFFFull - i;
-    kmax0 = (ka | hi) > kmax0 ? (ka | hi) : kmax0;
-    kmax1 = (kb | hi) > kmax1 ? (kb | hi) : kmax1;
-    kmin0 = (ka | i) < kmin0 ? (ka | i) : kmin0;
-    kmin1 = (kb | i) < kmin1 ? (kb | i) : kmin1;
+    // a NaN compares neither greater nor smaller: the scan's first element
+    // is kept if it is NaN, a NaN anywhere else is never picked
+    const bool na = va != va, nb = vb != vb;
+    const unsigned long long xa = na ? (i == 0 ? ~0ull : 0ull) : (ka | hi);
+    const unsigned long long xb = nb ? (i == 0 ? ~0ull : 0ull) : (kb | hi);
+    const unsigned long long ya = na ? (i == 0 ? 0ull : ~0ull) : (ka | i);
+    const unsigned long long yb = nb ? (i == 0 ? 0ull : ~0ull) : (kb | i);
+    kmax0 = xa > kmax0 ? xa : kmax0;
+    kmax1 = xb > kmax1 ? xb : kmax1;
+    kmin0 = ya < kmin0 ? ya : kmin0;
+    kmin1 = yb < kmin1 ? yb : kmin1;
 }
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)

@@ -58,6 +58,28 @@ def test_hs_640x480_single_scale_matches_oracle():
     assert np.array_equal(u, ou) and np.array_equal(v, ov)
 
 
+@pytest.mark.parametrize("levels", [1, 2, 3, 4])
+def test_hs_every_level_count_matches_oracle(levels):
+    """Every level but the last swaps the solver's flow buffers, so the host
+    entry point must return the final flow after an odd and an even number
+    of swaps; 96x80 also pads the coarse levels' rows (48 -> stride 64)."""
+    import torch
+    w, h = 96, 80
+    I0, I1, _, _ = synth.make_pair(w, h, 3.0, 7, bumps=True)
+    ou, ov = O.hs_compute(I0, I1, 15.0, levels, 2, 30)
+    with HornSchunck(levels, w, w, h) as hs:
+        u, v = hs.ComputeFlow(I0, I1, w, h, w, 15.0, levels, 2, 30)
+        assert np.array_equal(u, ou) and np.array_equal(v, ov)
+        d0 = torch.from_numpy(np.ascontiguousarray(I0, np.float32)).cuda()
+        d1 = torch.from_numpy(np.ascontiguousarray(I1, np.float32)).cuda()
+        du = torch.empty(w * h, dtype=torch.float32, device="cuda")
+        dv = torch.empty(w * h, dtype=torch.float32, device="cuda")
+        hs.compute_device(d0.data_ptr(), d1.data_ptr(), du.data_ptr(), dv.data_ptr(), 15.0, levels,
+                          2, 30)
+        assert np.array_equal(du.cpu().numpy().reshape(h, w), ou)
+        assert np.array_equal(dv.cpu().numpy().reshape(h, w), ov)
+
+
 @pytest.mark.parametrize("shape", [(1, 8), (4, 4), (8, 8)])
 def test_hs_multilevel_strided_matches_oracle(shape):
     w, h = 300, 200
