@@ -508,6 +508,147 @@ __global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__
     }
 }
 
+// kk whole iterations on a 64 x 64 tile in one launch (temporal blocking):
+// the iteration at a pixel reads the flow of every pixel of its window, so
+// after t iterations a tile's flow is exact at distance >= t*WC from its
+// edge, and the tile stores its central (64 - 2*kk*WC)^2 pixels.  Each of the
+// 1024 threads holds 4 pixels (rows r, r+16, r+32, r+48 of its lane's
+// column) with their flow, frame, gradient and tensor values in registers;
+// only the window products and their row sums go through LDS.  The
+// per-pixel arithmetic and both summation orders are k_lk_iter_fused's (the
+// windows clipped to the image), so the flow is the same bits as kk
+// launches of it.  For the small pyramid levels, where a launch costs more
+// than its iteration: the halo is recomputed (64^2 pixels per 34^2 stored at
+// WC = 3, kk = 5).
+constexpr int kLkBlockEdge = 64;
+
+template <int WC>
+__global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restrict__ P0,
+                                                           const float* __restrict__ P1,
+                                                           const float* __restrict__ Ix,
+                                                           const float* __restrict__ Iy,
+                                                           const float* __restrict__ G,
+                                                           const float* __restrict__ u,
+                                                           const float* __restrict__ v, int w,
+                                                           int h, float det_eps, int kk,
+                                                           float* __restrict__ uo,
+                                                           float* __restrict__ vo)
+{
+    constexpr int E = kLkBlockEdge;
+    constexpr int NP = E * E / 1024;   // pixels per thread
+    __shared__ float pr[2][E][E];
+    __shared__ float rs[2][E][E];
+    const int halo = kk * WC;
+    const int O = E - 2 * halo;
+    const int x0 = (int)blockIdx.x * O - halo, y0 = (int)blockIdx.y * O - halo;
+    const int ex = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t n = (size_t)w * h;
+    const int gx = x0 + ex;
+    const bool xin = gx >= 0 && gx < w;
+
+    float uu[NP], vv[NP], i0[NP], gxv[NP], gyv[NP], gxx[NP], gxy[NP], gyy[NP];
+    bool in[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        const int ey = wave + 16 * k, gy = y0 + ey;
+        in[k] = xin && gy >= 0 && gy < h;
+        const size_t p = in[k] ? (size_t)gy * w + gx : 0;
+        uu[k] = in[k] ? u[p] : 0.0f;
+        vv[k] = in[k] ? v[p] : 0.0f;
+        i0[k] = P0[p];
+        gxv[k] = Ix[p];
+        gyv[k] = Iy[p];
+        gxx[k] = G[p];
+        gxy[k] = G[n + p];
+        gyy[k] = G[2 * n + p];
+    }
+    for (int it = 0; it < kk; it++) {
+        const int lo = it * WC, hi = E - it * WC;   // flow exact here before this iteration
+        const int lo1 = lo + WC, hi1 = hi - WC;     // and here after it
+        // the window products on [lo, hi)^2, as k_lk_iter_fused forms them
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const int ey = wave + 16 * k, gy = y0 + ey;
+            if (ey < lo || ey >= hi || ex < lo || ex >= hi) continue;
+            const float x = (float)gx + uu[k], y = (float)gy + vv[k];
+            const float fx0 = floorf(x), fy0 = floorf(y);
+            const float ax = x - fx0, ay = y - fy0;
+            const int xi = clampi(f2i_sat(fx0), -1, w), yi = clampi(f2i_sat(fy0), -1, h);
+            const int xa = clampi(xi, 0, w - 1), xb = clampi(xi + 1, 0, w - 1);
+            const int ya = clampi(yi, 0, h - 1), yb = clampi(yi + 1, 0, h - 1);
+            const float ta = P1[(size_t)ya * w + xa], tb = P1[(size_t)ya * w + xb];
+            const float tc = P1[(size_t)yb * w + xa], td = P1[(size_t)yb * w + xb];
+            const float top = (1.0f - ax) * ta + ax * tb;
+            const float bot = (1.0f - ax) * tc + ax * td;
+            const float wv = (1.0f - ay) * top + ay * bot;
+            const float itv = wv - i0[k];
+            pr[0][ey][ex] = in[k] ? gxv[k] * itv : 0.0f;
+            pr[1][ey][ex] = in[k] ? gyv[k] * itv : 0.0f;
+        }
+        __syncthreads();
+        // row sums for rows [lo, hi), columns [lo1, hi1)
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const int ey = wave + 16 * k;
+            if (ey < lo || ey >= hi || ex < lo1 || ex >= hi1) continue;
+            float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+            for (int q = -WC; q <= WC; q++) {
+                const int jj = gx + q;
+                if (jj >= 0 && jj < w) {
+                    s0 += pr[0][ey][ex + q];
+                    s1 += pr[1][ey][ex + q];
+                }
+            }
+            rs[0][ey][ex] = s0;
+            rs[1][ey][ex] = s1;
+        }
+        __syncthreads();
+        // column sums and the 2x2 update on [lo1, hi1)^2
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const int ey = wave + 16 * k, gy = y0 + ey;
+            if (!in[k] || ey < lo1 || ey >= hi1 || ex < lo1 || ex >= hi1) continue;
+            float bx = 0.0f, by = 0.0f;
+#pragma unroll
+            for (int q = -WC; q <= WC; q++) {
+                const int ii = gy + q;
+                if (ii >= 0 && ii < h) {
+                    bx += rs[0][ey + q][ex];
+                    by += rs[1][ey + q][ex];
+                }
+            }
+            const float det = gxx[k] * gyy[k] - gxy[k] * gxy[k];
+            if (det > det_eps) {
+                const float du = (gxy[k] * by - gyy[k] * bx) / det;
+                const float dv = (gxy[k] * bx - gxx[k] * by) / det;
+                uu[k] = uu[k] + du;
+                vv[k] = vv[k] + dv;
+            }
+        }
+        // the next iteration's products overwrite pr only after every thread
+        // has passed the row sums (the barrier above); rs is overwritten
+        // after the barrier that follows those products
+    }
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        const int ey = wave + 16 * k, gy = y0 + ey;
+        if (!in[k] || ey < halo || ey >= E - halo || ex < halo || ex >= E - halo) continue;
+        const size_t p = (size_t)gy * w + gx;
+        uo[p] = uu[k];
+        vo[p] = vv[k];
+    }
+}
+
+// Iterations per launch of k_lk_iters_blocked: the stored centre keeps at
+// least 32 pixels per side.
+constexpr int lk_block_iters(int wc) { return (kLkBlockEdge - 32) / (2 * wc); }
+
+// Levels of at most this many pixels iterate in blocks (k_lk_iters_blocked);
+// larger ones one launch per iteration (k_lk_iter_fused), where the blocks'
+// recomputed halo would cost more than the launches it saves.
+constexpr long long kLkBlockedMaxPixels = 300000;
+
 // One pyramid step of both frames per launch (blockIdx.z = frame): the
 // Gaussian rows, the Gaussian columns, then zoom_out's resampling, with the
 // pixel bodies of the TV-L1 pyramid (mask.cpp:204-274, zoom.cpp:28-52).
@@ -732,7 +873,26 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
                                1);
             LK_CHECK("k_lk_boxsum");
         }
-        for (int it = 0; it < iters && wc >= 1 && wc <= 4; it++) {
+        // small levels: blocks of iterations per launch (k_lk_iters_blocked)
+        const bool blocked = wc >= 1 && wc <= 4 && (long long)w * h <= kLkBlockedMaxPixels;
+        for (int it = 0; it < iters && blocked;) {
+            const int kb = lk_block_iters(wc), kk = iters - it < kb ? iters - it : kb;
+            const int O = kLkBlockEdge - 2 * kk * wc;
+            const dim3 bgrid(cdiv(w, O), cdiv(h, O));
+            if (direct && l == 0 && it + kk == iters) {
+                nu = du;
+                nv = dv;
+            }
+            if (wc == 1) hipLaunchKernelGGL((k_lk_iters_blocked<1>), bgrid, dim3(1024), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, kk, nu, nv);
+            if (wc == 2) hipLaunchKernelGGL((k_lk_iters_blocked<2>), bgrid, dim3(1024), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, kk, nu, nv);
+            if (wc == 3) hipLaunchKernelGGL((k_lk_iters_blocked<3>), bgrid, dim3(1024), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, kk, nu, nv);
+            if (wc == 4) hipLaunchKernelGGL((k_lk_iters_blocked<4>), bgrid, dim3(1024), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, kk, nu, nv);
+            LK_CHECK("k_lk_iters_blocked");
+            float* t = u; u = nu; nu = t;
+            t = v; v = nv; nv = t;
+            it += kk;
+        }
+        for (int it = 0; it < iters && wc >= 1 && wc <= 4 && !blocked; it++) {
             // the whole iteration in one kernel, (u, v) -> (nu, nv)
             if (direct && l == 0 && it == iters - 1) {
                 nu = du;

@@ -114,6 +114,18 @@ def test_lk_single_scale_matches_oracle(spac, temp, win):
                 assert np.array_equal(Vy.reshape(h, w), oy)
 
 
+@pytest.mark.parametrize("w,h,win,iters", [(320, 240, 3, 1), (320, 240, 5, 5), (320, 240, 7, 5),
+                                           (333, 211, 7, 11), (333, 211, 9, 6), (97, 61, 3, 17),
+                                           (640, 360, 7, 5)])
+def test_lk_pyramidal_blocked_levels_match_oracle(w, h, win, iters):
+    """Levels of at most 300,000 pixels run blocks of iterations per launch
+    (several blocks when the count exceeds one block's; ragged tile edges)."""
+    I0, I1, _, _ = synth.make_pair(w, h, 4.0, 31 + win + iters)
+    u, v = lk_pyramidal(I0, I1, levels=3, zf=0.5, sigma=1.0, window=win, iters=iters, det_eps=1e-6)
+    ou, ov = O.lk_pyr(I0, I1, levels=3, zf=0.5, sigma=1.0, win=win, iters=iters, det_eps=1e-6)
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+
+
 def test_lk_pyramidal_1280x720_matches_oracle():
     """The LK workload: 1280x720, |flow| <= 8 px, 3 levels, 7x7, 5 iterations/level."""
     cfg = synth.CONFIGS["lk_1280x720"]
