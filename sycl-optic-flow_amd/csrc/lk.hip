@@ -358,168 +358,19 @@ __global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ row
     }
 }
 
-// One whole iteration on a 64 x kLkTileRows output tile: the warp and the two
-// products on the tile plus a WC-pixel halo (LDS), their row sums, then the
-// column sums and the 2x2 update.  Same expressions and summation order as
-// k_lk_it_products / k_lk_boxsum / k_lk_update (products outside the image
-// are never added: the windows are clipped to the image); reads (u, v),
-// writes the updated flow to (uo, vo).  Every global load a thread needs is
-// issued before the first use: the output pixels' flow and tensor, then the
-// extended tile's flow, then its samples, so one launch waits on two memory
-// round trips, not one per element.
-template <int WC, int TH = kLkTileRows>
-__global__ __launch_bounds__(256) void k_lk_iter_fused(const float* __restrict__ P0,
-                                                       const float* __restrict__ P1,
-                                                       const float* __restrict__ Ix,
-                                                       const float* __restrict__ Iy,
-                                                       const float* __restrict__ G,
-                                                       const float* __restrict__ u,
-                                                       const float* __restrict__ v, int w, int h,
-                                                       float det_eps, float* __restrict__ uo,
-                                                       float* __restrict__ vo)
-{
-    constexpr int TW = 64, EW = TW + 2 * WC, EH = TH + 2 * WC;
-    constexpr int NE = (EW * EH + 255) / 256;   // extended-tile elements per thread
-    constexpr int NO = TW * TH / 256;           // output pixels per thread
-    __shared__ float pr[2][EH][EW];
-    __shared__ float rs[2][EH][TW];
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const size_t n = (size_t)w * h;
-
-    float ou[NO], ov[NO], gxx[NO], gxy[NO], gyy[NO];
-#pragma unroll
-    for (int k = 0; k < NO; k++) {
-        const int e = threadIdx.x + 256 * k;
-        const int oy = e / TW, ox = e - oy * TW;
-        const int gx = x0 + ox, gy = y0 + oy;
-        ou[k] = ov[k] = gxx[k] = gxy[k] = gyy[k] = 0.0f;
-        if (gx < w && gy < h) {
-            const size_t p = (size_t)gy * w + gx;
-            ou[k] = u[p];
-            ov[k] = v[p];
-            gxx[k] = G[p];
-            gxy[k] = G[n + p];
-            gyy[k] = G[2 * n + p];
-        }
-    }
-
-    float eu[NE], ev[NE];
-#pragma unroll
-    for (int k = 0; k < NE; k++) {
-        const int e = threadIdx.x + 256 * k;
-        const int ey = e / EW, ex = e - ey * EW;
-        const int gx = x0 - WC + ex, gy = y0 - WC + ey;
-        eu[k] = ev[k] = 0.0f;
-        if (e < EW * EH && gx >= 0 && gx < w && gy >= 0 && gy < h) {
-            const size_t p = (size_t)gy * w + gx;
-            eu[k] = u[p];
-            ev[k] = v[p];
-        }
-    }
-    // every element's loads first, without branches (outside the image: a
-    // valid address, result discarded), so that they are all in flight
-    // together; then the arithmetic, as bilinear_at evaluates it
-    float ta[NE], tb[NE], tc[NE], td[NE], fx[NE], fy[NE], i0[NE], gxv[NE], gyv[NE];
-    bool in[NE];
-#pragma unroll
-    for (int k = 0; k < NE; k++) {
-        const int e = threadIdx.x + 256 * k;
-        const int ey = e / EW, ex = e - ey * EW;
-        const int gx = x0 - WC + ex, gy = y0 - WC + ey;
-        in[k] = e < EW * EH && gx >= 0 && gx < w && gy >= 0 && gy < h;
-        const size_t p = in[k] ? (size_t)gy * w + gx : 0;
-        const float x = (float)gx + eu[k], y = (float)gy + ev[k];
-        const float fx0 = floorf(x), fy0 = floorf(y);
-        fx[k] = x - fx0;
-        fy[k] = y - fy0;
-        const int xi = clampi(f2i_sat(fx0), -1, w), yi = clampi(f2i_sat(fy0), -1, h);
-        const int xa = clampi(xi, 0, w - 1), xb = clampi(xi + 1, 0, w - 1);
-        const int ya = clampi(yi, 0, h - 1), yb = clampi(yi + 1, 0, h - 1);
-        ta[k] = P1[(size_t)ya * w + xa];
-        tb[k] = P1[(size_t)ya * w + xb];
-        tc[k] = P1[(size_t)yb * w + xa];
-        td[k] = P1[(size_t)yb * w + xb];
-        i0[k] = P0[p];
-        gxv[k] = Ix[p];
-        gyv[k] = Iy[p];
-    }
-    float ea[NE], eb[NE];
-#pragma unroll
-    for (int k = 0; k < NE; k++) {
-        const float ax = fx[k], ay = fy[k];
-        const float top = (1.0f - ax) * ta[k] + ax * tb[k];
-        const float bot = (1.0f - ax) * tc[k] + ax * td[k];
-        const float wv = (1.0f - ay) * top + ay * bot;
-        const float itv = wv - i0[k];
-        ea[k] = in[k] ? gxv[k] * itv : 0.0f;
-        eb[k] = in[k] ? gyv[k] * itv : 0.0f;
-    }
-#pragma unroll
-    for (int k = 0; k < NE; k++) {
-        const int e = threadIdx.x + 256 * k;
-        if (e >= EW * EH) break;
-        const int ey = e / EW, ex = e - ey * EW;
-        pr[0][ey][ex] = ea[k];
-        pr[1][ey][ex] = eb[k];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < TW * EH; e += 256) {
-        const int ey = e / TW, ox = e - ey * TW;
-        const int gx = x0 + ox;
-        float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-        for (int k = -WC; k <= WC; k++) {
-            const int jj = gx + k;
-            if (jj >= 0 && jj < w) {
-                s0 += pr[0][ey][ox + WC + k];
-                s1 += pr[1][ey][ox + WC + k];
-            }
-        }
-        rs[0][ey][ox] = s0;
-        rs[1][ey][ox] = s1;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NO; k++) {
-        const int e = threadIdx.x + 256 * k;
-        const int oy = e / TW, ox = e - oy * TW;
-        const int gx = x0 + ox, gy = y0 + oy;
-        if (gx >= w || gy >= h) continue;
-        float bx = 0.0f, by = 0.0f;
-#pragma unroll
-        for (int kk = -WC; kk <= WC; kk++) {
-            const int ii = gy + kk;
-            if (ii >= 0 && ii < h) {
-                bx += rs[0][oy + WC + kk][ox];
-                by += rs[1][oy + WC + kk][ox];
-            }
-        }
-        const size_t p = (size_t)gy * w + gx;
-        const float det = gxx[k] * gyy[k] - gxy[k] * gxy[k];
-        float nu = ou[k], nv = ov[k];
-        if (det > det_eps) {
-            const float du = (gxy[k] * by - gyy[k] * bx) / det;
-            const float dv = (gxy[k] * bx - gxx[k] * by) / det;
-            nu = nu + du;
-            nv = nv + dv;
-        }
-        uo[p] = nu;
-        vo[p] = nv;
-    }
-}
-
 // kk whole iterations on a 64 x 64 tile in one launch (temporal blocking):
 // the iteration at a pixel reads the flow of every pixel of its window, so
 // after t iterations a tile's flow is exact at distance >= t*WC from its
-// edge, and the tile stores its central (64 - 2*kk*WC)^2 pixels.  Each of the
-// 1024 threads holds 4 pixels (rows r, r+16, r+32, r+48 of its lane's
-// column) with their flow, frame, gradient and tensor values in registers;
-// only the window products and their row sums go through LDS.  The
-// per-pixel arithmetic and both summation orders are k_lk_iter_fused's (the
-// windows clipped to the image), so the flow is the same bits as kk
-// launches of it.  For the small pyramid levels, where a launch costs more
-// than its iteration: the halo is recomputed (64^2 pixels per 34^2 stored at
-// WC = 3, kk = 5).
+// edge, and the tile stores its central (64 - 2*kk*WC)^2 pixels.  Wave r of
+// the 16 holds tile rows 4r .. 4r+3 (lane = column) with their flow, frame,
+// gradient and tensor values in registers.  A row of window products lies
+// in one wave, so the row sums take their neighbours through DPP lane
+// shifts; the column sums read the 2*WC rows next to a thread's four from
+// LDS (row sums, double-buffered: one barrier per iteration).  The
+// per-pixel arithmetic and both summation orders are those of the
+// three-kernel iteration k_lk_it_products / k_lk_boxsum / k_lk_update (the
+// windows clipped to the image), so the flow is the same bits as kk rounds
+// of it.
 constexpr int kLkBlockEdge = 64;
 
 template <int WC>
@@ -535,22 +386,24 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
                                                            float* __restrict__ vo)
 {
     constexpr int E = kLkBlockEdge;
-    constexpr int NP = E * E / 1024;   // pixels per thread
-    __shared__ float pr[2][E][E];
-    __shared__ float rs[2][E][E];
+    constexpr int NP = E * E / 1024;   // rows per thread
+    __shared__ float rs[2][2][E][E];   // [buffer][plane][row][column]
     const int halo = kk * WC;
     const int O = E - 2 * halo;
     const int x0 = (int)blockIdx.x * O - halo, y0 = (int)blockIdx.y * O - halo;
-    const int ex = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ex = threadIdx.x & 63, ey0 = (threadIdx.x >> 6) * NP;
     const size_t n = (size_t)w * h;
     const int gx = x0 + ex;
     const bool xin = gx >= 0 && gx < w;
+    bool cin[2 * WC + 1];   // column gx + q inside the image
+#pragma unroll
+    for (int q = -WC; q <= WC; q++) cin[q + WC] = gx + q >= 0 && gx + q < w;
 
     float uu[NP], vv[NP], i0[NP], gxv[NP], gyv[NP], gxx[NP], gxy[NP], gyy[NP];
     bool in[NP];
 #pragma unroll
     for (int k = 0; k < NP; k++) {
-        const int ey = wave + 16 * k, gy = y0 + ey;
+        const int gy = y0 + ey0 + k;
         in[k] = xin && gy >= 0 && gy < h;
         const size_t p = in[k] ? (size_t)gy * w + gx : 0;
         uu[k] = in[k] ? u[p] : 0.0f;
@@ -563,59 +416,88 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
         gyy[k] = G[2 * n + p];
     }
     for (int it = 0; it < kk; it++) {
-        const int lo = it * WC, hi = E - it * WC;   // flow exact here before this iteration
-        const int lo1 = lo + WC, hi1 = hi - WC;     // and here after it
-        // the window products on [lo, hi)^2, as k_lk_iter_fused forms them
+        const int lo1 = (it + 1) * WC, hi1 = E - (it + 1) * WC;   // flow exact here after it
+        const int b = it & 1;
+        // the window products of all four rows (k_lk_it_products'); the taps
+        // are clamped into the frame, so every gather is in bounds and all of
+        // them are issued before the first is used.  Values outside the
+        // square where the flow is exact are computed and never used.
+        float ta[NP], tb[NP], tc[NP], td[NP], ax[NP], ay[NP];
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-            const int ey = wave + 16 * k, gy = y0 + ey;
-            if (ey < lo || ey >= hi || ex < lo || ex >= hi) continue;
+            const int gy = y0 + ey0 + k;
             const float x = (float)gx + uu[k], y = (float)gy + vv[k];
             const float fx0 = floorf(x), fy0 = floorf(y);
-            const float ax = x - fx0, ay = y - fy0;
+            ax[k] = x - fx0;
+            ay[k] = y - fy0;
             const int xi = clampi(f2i_sat(fx0), -1, w), yi = clampi(f2i_sat(fy0), -1, h);
             const int xa = clampi(xi, 0, w - 1), xb = clampi(xi + 1, 0, w - 1);
             const int ya = clampi(yi, 0, h - 1), yb = clampi(yi + 1, 0, h - 1);
-            const float ta = P1[(size_t)ya * w + xa], tb = P1[(size_t)ya * w + xb];
-            const float tc = P1[(size_t)yb * w + xa], td = P1[(size_t)yb * w + xb];
-            const float top = (1.0f - ax) * ta + ax * tb;
-            const float bot = (1.0f - ax) * tc + ax * td;
-            const float wv = (1.0f - ay) * top + ay * bot;
-            const float itv = wv - i0[k];
-            pr[0][ey][ex] = in[k] ? gxv[k] * itv : 0.0f;
-            pr[1][ey][ex] = in[k] ? gyv[k] * itv : 0.0f;
+            ta[k] = P1[(size_t)ya * w + xa];
+            tb[k] = P1[(size_t)ya * w + xb];
+            tc[k] = P1[(size_t)yb * w + xa];
+            td[k] = P1[(size_t)yb * w + xb];
         }
-        __syncthreads();
-        // row sums for rows [lo, hi), columns [lo1, hi1)
+        float r0[NP], r1[NP];   // row sums of this thread's rows
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-            const int ey = wave + 16 * k;
-            if (ey < lo || ey >= hi || ex < lo1 || ex >= hi1) continue;
+            const float top = (1.0f - ax[k]) * ta[k] + ax[k] * tb[k];
+            const float bot = (1.0f - ax[k]) * tc[k] + ax[k] * td[k];
+            const float wv = (1.0f - ay[k]) * top + ay[k] * bot;
+            const float itv = wv - i0[k];
+            float a[2 * WC + 1], c[2 * WC + 1];   // products at columns ex - WC .. ex + WC
+            a[WC] = in[k] ? gxv[k] * itv : 0.0f;
+            c[WC] = in[k] ? gyv[k] * itv : 0.0f;
+#pragma unroll
+            for (int d = 1; d <= WC; d++) {
+                a[WC + d] = from_right(a[WC + d - 1]);
+                c[WC + d] = from_right(c[WC + d - 1]);
+                a[WC - d] = from_left(a[WC - d + 1]);
+                c[WC - d] = from_left(c[WC - d + 1]);
+            }
             float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
-            for (int q = -WC; q <= WC; q++) {
-                const int jj = gx + q;
-                if (jj >= 0 && jj < w) {
-                    s0 += pr[0][ey][ex + q];
-                    s1 += pr[1][ey][ex + q];
+            for (int q = 0; q <= 2 * WC; q++) {
+                if (cin[q]) {
+                    s0 += a[q];
+                    s1 += c[q];
                 }
             }
-            rs[0][ey][ex] = s0;
-            rs[1][ey][ex] = s1;
+            r0[k] = s0;
+            r1[k] = s1;
+            rs[b][0][ey0 + k][ex] = s0;
+            rs[b][1][ey0 + k][ex] = s1;
         }
         __syncthreads();
-        // column sums and the 2x2 update on [lo1, hi1)^2
+        // column sums of rows ey0 - WC .. ey0 + NP - 1 + WC: the thread's own
+        // from registers, the others from LDS (rows clamped into the tile are
+        // read only for pixels outside the exact square)
+        float y0s[NP + 2 * WC], y1s[NP + 2 * WC];
+#pragma unroll
+        for (int j = 0; j < WC; j++) {
+            const int ra = ey0 - WC + j < 0 ? 0 : ey0 - WC + j;
+            const int rb = ey0 + NP + j > E - 1 ? E - 1 : ey0 + NP + j;
+            y0s[j] = rs[b][0][ra][ex];
+            y1s[j] = rs[b][1][ra][ex];
+            y0s[WC + NP + j] = rs[b][0][rb][ex];
+            y1s[WC + NP + j] = rs[b][1][rb][ex];
+        }
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-            const int ey = wave + 16 * k, gy = y0 + ey;
+            y0s[WC + k] = r0[k];
+            y1s[WC + k] = r1[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const int ey = ey0 + k, gy = y0 + ey;
             if (!in[k] || ey < lo1 || ey >= hi1 || ex < lo1 || ex >= hi1) continue;
             float bx = 0.0f, by = 0.0f;
 #pragma unroll
             for (int q = -WC; q <= WC; q++) {
                 const int ii = gy + q;
                 if (ii >= 0 && ii < h) {
-                    bx += rs[0][ey + q][ex];
-                    by += rs[1][ey + q][ex];
+                    bx += y0s[WC + k + q];
+                    by += y1s[WC + k + q];
                 }
             }
             const float det = gxx[k] * gyy[k] - gxy[k] * gxy[k];
@@ -626,13 +508,13 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
                 vv[k] = vv[k] + dv;
             }
         }
-        // the next iteration's products overwrite pr only after every thread
-        // has passed the row sums (the barrier above); rs is overwritten
-        // after the barrier that follows those products
+        // rs[b] is written again two iterations on, after the barrier of
+        // the next iteration, which every thread reaches only once its reads
+        // of rs[b] above are done
     }
 #pragma unroll
     for (int k = 0; k < NP; k++) {
-        const int ey = wave + 16 * k, gy = y0 + ey;
+        const int ey = ey0 + k, gy = y0 + ey;
         if (!in[k] || ey < halo || ey >= E - halo || ex < halo || ex >= E - halo) continue;
         const size_t p = (size_t)gy * w + gx;
         uo[p] = uu[k];
@@ -640,14 +522,22 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
     }
 }
 
-// Iterations per launch of k_lk_iters_blocked: the stored centre keeps at
-// least 32 pixels per side.
-constexpr int lk_block_iters(int wc) { return (kLkBlockEdge - 32) / (2 * wc); }
-
-// Levels of at most this many pixels iterate in blocks (k_lk_iters_blocked);
-// larger ones one launch per iteration (k_lk_iter_fused), where the blocks'
-// recomputed halo would cost more than the launches it saves.
-constexpr long long kLkBlockedMaxPixels = 300000;
+// Iterations per launch of k_lk_iters_blocked.  A launch stores the centre
+// (64 - 2*kk*WC)^2 of each 64 x 64 tile, so (64 / (64 - 2*kk*WC))^2 pixels
+// are read and iterated per stored pixel.  Levels of at most kLkSmallLevel
+// pixels have too few tiles to fill the GPU and take the deepest blocks (a
+// centre of at least 32 pixels per side); larger levels keep the halo within
+// 9 pixels (1.9 pixels iterated per stored one at WC = 3, kk = 3).  On 1280 x
+// 720 frames, 3 levels and 7 x 7 windows the solve took 0.241 ms with one
+// launch per iteration on every level, 0.226 ms with blocks on the two small
+// levels only, 0.212 ms with level 0 in blocks of 3 and 0.244 ms in blocks
+// of 5 (blocks of 2 were slower than blocks of 3 in an earlier version of
+// the kernel).
+constexpr long long kLkSmallLevel = 300000;
+constexpr int lk_block_iters(int wc, long long npix)
+{
+    return npix <= kLkSmallLevel ? (kLkBlockEdge - 32) / (2 * wc) : (9 / wc > 1 ? 9 / wc : 1);
+}
 
 // One pyramid step of both frames per launch (blockIdx.z = frame): the
 // Gaussian rows, the Gaussian columns, then zoom_out's resampling, with the
@@ -833,7 +723,7 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
         LK_CHECK("k_lk_pyr_zoom");
     }
     const int wc = (win - 1) / 2;
-    // the last fused iteration at level 0 writes the caller's du/dv
+    // the last iteration launch at level 0 writes the caller's du/dv
     // directly unless they overlap the frames or each other
     // address ranges of len floats, compared as integers (the buffers
     // are separate allocations)
@@ -873,10 +763,11 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
                                1);
             LK_CHECK("k_lk_boxsum");
         }
-        // small levels: blocks of iterations per launch (k_lk_iters_blocked)
-        const bool blocked = wc >= 1 && wc <= 4 && (long long)w * h <= kLkBlockedMaxPixels;
+        // windows up to 9 x 9: blocks of iterations per launch (k_lk_iters_blocked)
+        const bool blocked = wc >= 1 && wc <= 4;
         for (int it = 0; it < iters && blocked;) {
-            const int kb = lk_block_iters(wc), kk = iters - it < kb ? iters - it : kb;
+            const int kb = lk_block_iters(wc, (long long)w * h);
+            const int kk = iters - it < kb ? iters - it : kb;
             const int O = kLkBlockEdge - 2 * kk * wc;
             const dim3 bgrid(cdiv(w, O), cdiv(h, O));
             if (direct && l == 0 && it + kk == iters) {
@@ -892,21 +783,7 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
             t = v; v = nv; nv = t;
             it += kk;
         }
-        for (int it = 0; it < iters && wc >= 1 && wc <= 4 && !blocked; it++) {
-            // the whole iteration in one kernel, (u, v) -> (nu, nv)
-            if (direct && l == 0 && it == iters - 1) {
-                nu = du;
-                nv = dv;
-            }
-            if (wc == 1) hipLaunchKernelGGL((k_lk_iter_fused<1>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-            if (wc == 2) hipLaunchKernelGGL((k_lk_iter_fused<2>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-            if (wc == 3) hipLaunchKernelGGL((k_lk_iter_fused<3>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-            if (wc == 4) hipLaunchKernelGGL((k_lk_iter_fused<4>), fgrid, dim3(256), 0, st, p0, p1, Ix, Iy, GT, u, v, w, h, det_eps, nu, nv);
-            LK_CHECK("k_lk_iter_fused");
-            float* t = u; u = nu; nu = t;
-            t = v; v = nv; nv = t;
-        }
-        for (int it = 0; it < iters && (wc < 1 || wc > 4); it++) {
+        for (int it = 0; it < iters && !blocked; it++) {
             hipLaunchKernelGGL(k_lk_it_products, grid2d(w, h), dim3(256), 0, st, p0, p1, Ix, Iy,
                                u, v, w, h, PROD);
             LK_CHECK("k_lk_it_products");

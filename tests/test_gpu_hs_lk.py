@@ -116,10 +116,14 @@ def test_lk_single_scale_matches_oracle(spac, temp, win):
 
 @pytest.mark.parametrize("w,h,win,iters", [(320, 240, 3, 1), (320, 240, 5, 5), (320, 240, 7, 5),
                                            (333, 211, 7, 11), (333, 211, 9, 6), (97, 61, 3, 17),
-                                           (640, 360, 7, 5)])
+                                           (640, 360, 7, 5), (801, 401, 3, 10), (721, 433, 9, 5),
+                                           (700, 451, 5, 9)])
 def test_lk_pyramidal_blocked_levels_match_oracle(w, h, win, iters):
-    """Levels of at most 300,000 pixels run blocks of iterations per launch
-    (several blocks when the count exceeds one block's; ragged tile edges)."""
+    """Windows up to 9 x 9 run blocks of iterations per launch: levels of at
+    most 300,000 pixels the deepest blocks, larger levels blocks with a halo
+    of at most 9 pixels (9, 4 and 2 iterations at windows 3, 5 and 9); a
+    count beyond one block's takes several; ragged tile edges; window 11
+    runs the three-kernel iteration."""
     I0, I1, _, _ = synth.make_pair(w, h, 4.0, 31 + win + iters)
     u, v = lk_pyramidal(I0, I1, levels=3, zf=0.5, sigma=1.0, window=win, iters=iters, det_eps=1e-6)
     ou, ov = O.lk_pyr(I0, I1, levels=3, zf=0.5, sigma=1.0, win=win, iters=iters, det_eps=1e-6)
@@ -138,7 +142,7 @@ def test_lk_pyramidal_1280x720_matches_oracle():
 @pytest.mark.parametrize("levels,win,iters", [(1, 5, 3), (2, 9, 4), (4, 7, 2), (2, 3, 3), (2, 11, 2),
                                                (1, 1, 2)])
 def test_lk_pyramidal_variants_match_oracle(levels, win, iters):
-    """Windows 3-9 run the fused iteration kernel, 1 and 11 the three-kernel
+    """Windows 3-9 run the blocked iteration kernel, 1 and 11 the three-kernel
     path; both match the oracle bit for bit."""
     I0, I1, _, _ = synth.make_pair(257, 190, 3.0, 22 + levels)
     u, v = lk_pyramidal(I0, I1, levels=levels, window=win, iters=iters)

@@ -2,7 +2,7 @@
 per-GPU scratch, no allocation) against of_lk_pyr_compute_device, called
 through the C ABI: bit-identical flow on two calls in a row, the second on a
 smaller input (the scratch then still holds the first call's data), both
-kernel paths (fused windows 3-9, three-kernel windows 1 and 11), and the
+kernel paths (blocked windows 3-9, three-kernel windows 1 and 11), and the
 capacity error for an input beyond the scratch."""
 import numpy as np
 import pytest
