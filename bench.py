@@ -608,7 +608,7 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4),
-        "kernel": ("k_tvl1_persist" if dom["persistent"] else "k_tvl1_iter")
+        "kernel": {0: "k_tvl1_iter", 1: "k_tvl1_persist", 2: "k_tvl1_persist_nbr"}[dom["persistent"]]
                   + f"<{dom['iters_per_launch']},{dom['rows_per_wave']}>",
         "bytes_per_launch": round(per_launch_bytes), "avg_launch_us": round(avg_launch_s * 1e6, 2),
         "launches": dom["launches"],

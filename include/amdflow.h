@@ -125,10 +125,13 @@ int of_tvl1_get_stats(const of_tvl1 *h, of_tvl1_stats *st);
  * iters_per_launch pixels.  Supported: iters_per_launch in {1,2,4,8},
  * rows_per_wave in {2,4,8}; rows_per_wave 0 selects square 16 x 16 tiles,
  * one pixel per lane (iters_per_launch in {1,2,4}; the built-in choice,
- * host-paced, below 200000 pixels).  `persistent` = 1 runs each warping of those
- * scales as one launch of a co-resident grid (chunks of iters_per_launch
- * iterations separated by grid barriers; see of_set_persist_launch_mode)
- * instead of host-paced launches.  Rules are
+ * host-paced, below 200000 pixels).  `persistent` = 0 runs host-paced launches;
+ * 1 runs each warping of those scales as one launch of a co-resident grid,
+ * chunks of iters_per_launch iterations separated by grid barriers; 2 runs
+ * the same single launch with neighbour flags instead of grid barriers (a
+ * tile starts a chunk when its 8 neighbours have finished the previous one,
+ * and one extra workgroup applies the stopping rule while the tiles go on;
+ * see of_set_persist_launch_mode); other values return OF_ERR_ARG.  Rules are
  * matched largest min_pixels first; calling with min_pixels < 0 clears all
  * rules (back to the built-in table). */
 int of_tvl1_set_launch_rule(of_tvl1 *h, long long min_pixels, int iters_per_launch,
@@ -153,19 +156,21 @@ int of_set_persist_launch_mode(int mode);
 int of_tvl1_set_inflight(of_tvl1 *h, int launches);
 
 /* Autotuner of the launch shapes (no reference counterpart): tries every
- * supported (iters_per_launch, rows_per_wave) in both launch modes
- * (host-paced, persistent) on all scales, times each scale with HIP events
- * over `reps` solves of the device pair dI0/dI1 (width*height bytes each)
- * and installs the fastest variant per scale as launch rules.  `chosen`
- * (optional) receives 3*nscales ints: K, R, persistent for each scale,
- * finest first; `ms_table` (optional) receives OF_TVL1_TUNE_VARIANTS (24)
- * x nscales mean times in ms: rows 0-8 host-paced, 9-17 persistent, shapes
- * in the order (1,2) (2,2) (1,4) (2,4) (4,4) (1,8) (2,8) (4,8) (8,8); rows
- * 18-20 square 16 x 16 tiles (rows_per_wave 0) host-paced with 1, 2 and 4
- * iterations per launch, rows 21-23 the same persistent; -1 marks a variant
- * the device cannot run.  Results are bit-identical for every variant; only
- * the time changes. */
-#define OF_TVL1_TUNE_VARIANTS 24
+ * supported (iters_per_launch, rows_per_wave) in the three launch modes
+ * (host-paced, persistent with grid barriers, persistent with neighbour
+ * flags) on all scales, times each scale with HIP events over `reps` solves
+ * of the device pair dI0/dI1 (width*height bytes each) and installs the
+ * fastest variant per scale as launch rules.  `chosen` (optional) receives
+ * 3*nscales ints: K, R, persistent for each scale, finest first; `ms_table`
+ * (optional) receives OF_TVL1_TUNE_VARIANTS (36) x nscales mean times in ms:
+ * rows 0-8 host-paced, 9-17 persistent with grid barriers, shapes in the
+ * order (1,2) (2,2) (1,4) (2,4) (4,4) (1,8) (2,8) (4,8) (8,8); rows 18-20
+ * square 16 x 16 tiles (rows_per_wave 0) host-paced with 1, 2 and 4
+ * iterations per launch, rows 21-23 the same persistent with grid barriers;
+ * rows 24-32 the nine shapes persistent with neighbour flags, rows 33-35
+ * the square tiles the same; -1 marks a variant the device cannot run.
+ * Results are bit-identical for every variant; only the time changes. */
+#define OF_TVL1_TUNE_VARIANTS 36
 int of_tvl1_autotune(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, int reps, int *chosen,
                      double *ms_table);
 
@@ -179,7 +184,7 @@ int of_tvl1_autotune(of_tvl1 *h, const uint8_t *dI0, const uint8_t *dI1, int rep
 typedef struct {
     int iters_per_launch;
     int rows_per_wave;
-    int persistent;                /* 1: whole-warping cooperative kernel    */
+    int persistent;                /* 1, 2: whole warping in one launch      */
     long long launches;            /* launches timed (including no-op ones) */
     long long pixel_iterations;    /* iterations completed x scale pixels    */
     double milliseconds;           /* summed launch durations                */

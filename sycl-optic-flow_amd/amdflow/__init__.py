@@ -299,20 +299,24 @@ class TV_L1:
         return {f: getattr(st, f) for f, _ in TVL1Stats._fields_}
 
     def set_launch_rule(self, min_pixels: int, iters_per_launch: int, rows_per_wave: int,
-                        persistent: bool = False) -> None:
+                        persistent: int = 0) -> None:
+        """persistent: 0 host-paced, 1 (or True) one launch per warping with
+        grid barriers, 2 one launch per warping with neighbour flags."""
         _check(_lib.of_tvl1_set_launch_rule(self._h, int(min_pixels), int(iters_per_launch),
-                                            int(rows_per_wave), int(bool(persistent))),
+                                            int(rows_per_wave), int(persistent)),
                "set_launch_rule")
 
     AUTOTUNE_SHAPES = [(1, 2), (2, 2), (1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8), (8, 8)]
     AUTOTUNE_VARIANTS = ([(k, r, 0) for k, r in AUTOTUNE_SHAPES] + [(k, r, 1) for k, r in AUTOTUNE_SHAPES]
                          + [(1, 0, 0), (2, 0, 0), (4, 0, 0)]      # square 16x16 tiles, host-paced
-                         + [(1, 0, 1), (2, 0, 1), (4, 0, 1)])     # and persistent
+                         + [(1, 0, 1), (2, 0, 1), (4, 0, 1)]      # and persistent
+                         + [(k, r, 2) for k, r in AUTOTUNE_SHAPES]  # neighbour flags
+                         + [(1, 0, 2), (2, 0, 2), (4, 0, 2)])
 
     def autotune(self, d_I0: int, d_I1: int, reps: int = 2):
         """Times every launch variant (K, R, persistent) per scale on a device u8
         pair and keeps the fastest per scale; returns (chosen [(K, R, P)] per
-        scale, ms table [18 x nscales])."""
+        scale, ms table [36 x nscales])."""
         ns = self.nscales
         chosen = np.zeros(3 * ns, np.int32)
         table = np.zeros(len(self.AUTOTUNE_VARIANTS) * ns, np.float64)

@@ -48,4 +48,4 @@ def apply(tv, shapes) -> None:
         k, r = shape[0], shape[1]
         p = shape[2] if len(shape) > 2 else 0
         nx, ny = tv.scale_size(s)
-        tv.set_launch_rule(nx * ny, int(k), int(r), bool(p))
+        tv.set_launch_rule(nx * ny, int(k), int(r), int(p))

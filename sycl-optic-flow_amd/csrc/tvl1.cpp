@@ -91,6 +91,7 @@ void release(of_tvl1* h)
     dev_free(h->init);
     dev_free(h->ctl);
     dev_free(h->partials);
+    dev_free(h->tile_flags);
     dev_free(h->energy);
     dev_free(h->d_iters);
     host_free(h->hflow);
@@ -111,7 +112,7 @@ int timed_launch(of_tvl1* h, int K, int R, int persistent, const IterArgs& a, hi
         h->ev_pool.push_back(e);
     }
     OF_HIP(hipEventRecord(h->ev_pool[ev], st));
-    if (persistent) OF_TRY(launch_persist(K, R, a, 0, st));
+    if (persistent) OF_TRY(launch_persist(K, R, a, 0, st, persistent));
     else OF_TRY(launch_iter(K, R, a, st));
     OF_HIP(hipEventRecord(h->ev_pool[ev + 1], st));
     h->timed.push_back({ev, K, R, persistent});
@@ -190,6 +191,8 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     a.partials = h->partials;
     a.host_status = h->hstatus;
     a.epoch = ++h->epoch;
+    a.tile_flags = h->tile_flags;
+    a.seq = next_warp_seq();
 
     if (persistent && persist_host_paced()) {
         persistent = 0;   // the process gave up persistent launches
@@ -197,9 +200,9 @@ int run_warping(of_tvl1* h, int s, int w, hipStream_t st)
     }
     if (persistent) {
         // the whole warping in one launch: no host involvement
-        if (h->profiling) return timed_launch(h, K, R, 1, a, st);
+        if (h->profiling) return timed_launch(h, K, R, persistent, a, st);
         h->launches_issued++;
-        return launch_persist(K, R, a, 0, st);
+        return launch_persist(K, R, a, 0, st, persistent);
     }
 
     const long long max_launches = (h->run_max + K - 1) / K + 1;
@@ -469,14 +472,19 @@ int autotune(of_tvl1* h, const uint8_t* d0, const uint8_t* d1, int reps, int* ch
     std::vector<double> best(h->ns, 1e30);
     std::vector<int> bestK(h->ns, 2), bestR(h->ns, 8), bestP(h->ns, 0);
     const std::vector<LaunchRule> saved = h->rules;
-    // modes: 64-column tiles host-paced, persistent; square tiles host-paced,
-    // persistent
-    for (int mode = 0; mode < 4; mode++) {
-        for (int c = 0; c < (mode < 2 ? nshape : nsquare); c++) {
-            const int row = mode < 2 ? mode * nshape + c : 2 * nshape + (mode - 2) * nsquare + c;
-            const int K = mode < 2 ? kShapes[c][0] : kSquareK[c];
-            const int R = mode < 2 ? kShapes[c][1] : 0;
-            const int P = (mode == 1 || mode == 3) ? 1 : 0;
+    // modes: 64-column tiles host-paced, persistent with grid barriers;
+    // square tiles host-paced, persistent with grid barriers; persistent
+    // with neighbour flags, 64-column tiles and square tiles
+    static const int kModeP[] = {kHostPaced, kPersistBarrier, kHostPaced, kPersistBarrier,
+                                 kPersistNeighbour, kPersistNeighbour};
+    static const bool kModeSquare[] = {false, false, true, true, false, true};
+    int row = 0;
+    for (int mode = 0; mode < 6; mode++) {
+        const bool square = kModeSquare[mode];
+        for (int c = 0; c < (square ? nsquare : nshape); c++, row++) {
+            const int K = square ? kSquareK[c] : kShapes[c][0];
+            const int R = square ? 0 : kShapes[c][1];
+            const int P = kModeP[mode];
             h->rules.assign(1, LaunchRule{0, K, R, P});
             int st = solve(h, d0, d1, true, h->dflow, hs);   // warm-up of this variant
             // a persistent variant whose grid barrier ran past its bound is
@@ -593,14 +601,14 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         if (st) { delete h; return st; }
     
         const size_t N0 = (size_t)width * height;
-        size_t maxpart = 1;
+        size_t maxpart = 1, maxtiles = 1;
         const int Ks[] = {1, 2, 4, 8}, Rs[] = {0, 2, 4, 8};
         for (int K : Ks)
             for (int R : Rs)
                 if (iter_config_ok(K, R)) {
-                    unsigned gx, gy;
-                    iter_grid(K, R, width, height, &gx, &gy);
-                    maxpart = std::max(maxpart, (size_t)gx * gy * K);
+                    const size_t t = iter_tiles(K, R, width, height);
+                    maxpart = std::max(maxpart, t * K);
+                    maxtiles = std::max(maxtiles, t);
                 }
     
         st = OF_OK;
@@ -621,6 +629,7 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         if (!st) st = dev_alloc(&h->ctl, 1);
         // x2: the persistent kernel double-buffers its partials by chunk parity
         if (!st) st = dev_alloc(&h->partials, 2 * maxpart);
+        if (!st) st = dev_alloc(&h->tile_flags, maxtiles);
         if (!st) st = dev_alloc(&h->energy, 2 * kEnergyBlocks + 3);
         if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
         if (!st) st = host_alloc(&h->hflow, 2 * N0, hipHostMallocDefault);
@@ -629,7 +638,9 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         *h->hstatus = 0ull;
         std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
         h->h_iters.assign((size_t)ns * p.warps, 0);
-        if (hipMemset(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps) != hipSuccess) {
+        // flags carry the warping's seq (from 1): zero means never set
+        if (hipMemset(h->d_iters, 0, sizeof(int) * (size_t)ns * p.warps) != hipSuccess ||
+            hipMemset(h->tile_flags, 0, sizeof(unsigned long long) * maxtiles) != hipSuccess) {
             release(h);
             return OF_ERR_HIP;
         }
@@ -1022,7 +1033,8 @@ int of_tvl1_set_launch_rule(of_tvl1* h, long long min_pixels, int iters_per_laun
             return OF_OK;
         }
         if (!iter_config_ok(iters_per_launch, rows_per_wave)) return OF_ERR_UNSUPPORTED;
-        const int p = persistent ? 1 : 0;
+        if (persistent < kHostPaced || persistent > kPersistNeighbour) return OF_ERR_ARG;
+        const int p = persistent;
         for (LaunchRule& r : h->rules)
             if (r.min_pixels == min_pixels) {
                 r.K = iters_per_launch;

@@ -20,7 +20,7 @@ struct LaunchRule {
     long long min_pixels;
     int K;
     int R;
-    int persistent;   // 1: whole warping in one persistent launch
+    int persistent;   // kHostPaced, kPersistBarrier or kPersistNeighbour
 };
 
 // Enqueues the first phase of a solve on `st`: frames to float (u8 or
@@ -58,6 +58,7 @@ struct of_tvl1 {
     float* hflow = nullptr;     // pinned host copy returned by of_tvl1_flow
     amdflow::TvCtl* ctl = nullptr;
     double* partials = nullptr;
+    unsigned long long* tile_flags = nullptr;   // one word per tile (neighbour-synchronised warpings)
     double* energy = nullptr;   // 2*kEnergyBlocks partials, then (E, data, TV)
     bool prepared = false;      // pyramids hold the last prepared pair
     bool solved = false;        // per-scale flows hold the last solve

@@ -79,6 +79,13 @@ __device__ __forceinline__ void st(float* p, float v)
     else *p = v;
 }
 
+// Called by every thread of a tile's workgroup between its iterations and
+// its stores: false discards the tile's results (the neighbour-synchronised
+// warping checks its speculative chunk there).  The default stores always.
+struct StoreAlways {
+    __device__ bool operator()() const { return true; }
+};
+
 // One LDS row per wave and exchanged quantity; NW waves per workgroup.
 template <int NW>
 struct TileSharedN {
@@ -96,9 +103,9 @@ using TileShared = TileSharedN<4>;
 // registers or, across waves, from one LDS row.  Values within K pixels of
 // the tile edge go stale one pixel per iteration; the central (64-2K) x
 // (NW*R-2K) pixels are exact.
-template <int K, int R, bool COH, bool INNER, int NW>
+template <int K, int R, bool COH, bool INNER, int NW, class Pre>
 __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int by, int cur, int kk,
-                                                  double (&acc)[K], TileSharedN<NW>& sh)
+                                                  double (&acc)[K], TileSharedN<NW>& sh, Pre& pre)
 {
     constexpr int TW = 64 - 2 * K;
     constexpr int TH = NW * R - 2 * K;
@@ -208,6 +215,7 @@ __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int
         }
     }
 
+    if (!pre()) return;
     float* uout = a.u[cur ^ 1];
     float* pout = a.p[cur ^ 1];
     if (xout) {
@@ -229,16 +237,16 @@ __device__ __forceinline__ void tile_iterate_impl(const IterArgs& a, int bx, int
 
 // Tiles whose extended area (64 columns x 4R rows from (bx*TW-K, by*TH-K))
 // stays clear of the image border run the INNER variant.
-template <int K, int R, bool COH, int NW>
+template <int K, int R, bool COH, int NW, class Pre = StoreAlways>
 __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, int cur, int kk,
-                                             double (&acc)[K], TileSharedN<NW>& sh)
+                                             double (&acc)[K], TileSharedN<NW>& sh, Pre pre = Pre{})
 {
     constexpr int TW = 64 - 2 * K;
     constexpr int TH = NW * R - 2 * K;
     const int x0 = bx * TW - K, y0 = by * TH - K;
     const bool inner = x0 >= 1 && x0 + 63 <= a.nx - 2 && y0 >= 1 && y0 + NW * R - 1 <= a.ny - 2;
-    if (inner) tile_iterate_impl<K, R, COH, true, NW>(a, bx, by, cur, kk, acc, sh);
-    else tile_iterate_impl<K, R, COH, false, NW>(a, bx, by, cur, kk, acc, sh);
+    if (inner) tile_iterate_impl<K, R, COH, true, NW>(a, bx, by, cur, kk, acc, sh, pre);
+    else tile_iterate_impl<K, R, COH, false, NW>(a, bx, by, cur, kk, acc, sh, pre);
 }
 
 // Square tiles for the small scales: one workgroup of 256 lanes covers a
@@ -259,9 +267,9 @@ struct SqShared {
 // term over the central tile to acc[t].  Same per-pixel expressions as
 // tile_iterate_impl; pixels outside the image never feed pixels inside (the
 // border rules do not read across the border).
-template <int K, bool COH>
+template <int K, bool COH, class Pre = StoreAlways>
 __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int by, int cur, int kk,
-                                                double (&acc)[K], SqShared& sh)
+                                                double (&acc)[K], SqShared& sh, Pre pre = Pre{})
 {
     constexpr int T = kSqTile, C = T - 2 * K;
     const int lx = threadIdx.x % T, ly = threadIdx.x / T;
@@ -336,6 +344,7 @@ __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int b
             p22 = (p22 + taut * fy2) / g2;
         }
     }
+    if (!pre()) return;
     if (mine) {
         float* uout = a.u[cur ^ 1];
         float* pout = a.p[cur ^ 1];
@@ -767,6 +776,240 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist(IterArgs a, unsigned tiles_x, 
 }
 
 // ---------------------------------------------------------------------------
+// neighbour-synchronised persistent driver
+// ---------------------------------------------------------------------------
+
+// Bounded wait of one thread until the 64-bit word at `w` reaches `want`
+// (the words only grow within a warping).  Returns false after flagging
+// kBarTimeout past ~1 s of polls, after the first bound (kBarNotResident),
+// or once another workgroup has flagged an error.
+__device__ __forceinline__ bool word_wait_ge(const unsigned long long* w, unsigned long long want,
+                                             TvCtl* ctl, const FirstBound& fb)
+{
+    const unsigned long long t_end = fb.ticks ? (unsigned long long)wall_clock64() + fb.ticks : 0ull;
+    if (fb.force) {
+        __hip_atomic_store(&ctl->error, kBarNotResident, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+    unsigned spins = 0;
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 21)) {
+            __hip_atomic_store(&ctl->error, kBarTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        if (past_first_bound(fb, t_end, spins, ctl)) return false;
+        if ((spins & 1023u) == 0 &&
+            __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+    }
+    return true;
+}
+
+// The referee's bounded wait for a chunk's counter to reach `want`.
+__device__ __forceinline__ bool count_wait(const unsigned* w, unsigned want, TvCtl* ctl)
+{
+    unsigned spins = 0;
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 21)) {
+            __hip_atomic_store(&ctl->error, kBarTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        if ((spins & 1023u) == 0 &&
+            __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+    }
+    return true;
+}
+
+// Tile (bx, by) may start chunk c once its (up to) 8 neighbours have
+// completed chunk c-1: its extended tile reads their centres, and its
+// stores overwrite values their chunk c-1 read.  Threads 0..8 poll one
+// neighbour each; the workgroup learns whether every wait ended normally.
+__device__ __forceinline__ bool wait_neighbours(const IterArgs& a, unsigned tiles_x, unsigned tiles_y,
+                                                int bx, int by, int c, const FirstBound& fb)
+{
+    int ok = 1;
+    const unsigned t = threadIdx.x;
+    if (t < 9 && t != 4) {
+        const int x = bx + (int)(t % 3) - 1, y = by + (int)(t / 3) - 1;
+        if (x >= 0 && y >= 0 && x < (int)tiles_x && y < (int)tiles_y)
+            ok = word_wait_ge(a.tile_flags + (size_t)y * tiles_x + x, (a.seq << 20) | (unsigned)c,
+                              a.ctl, fb);
+    }
+    return __syncthreads_and(ok) != 0;
+}
+
+// The stopping decision of chunk c as published by the referee workgroup:
+// keep (0: continue) in every thread, or -1 if the wait failed.
+__device__ __forceinline__ int wait_decision(const IterArgs& a, int c, int parity, const FirstBound& fb,
+                                             int* sh_keep)
+{
+    if (threadIdx.x == 0) {
+        const unsigned long long want = (a.seq << 24) | ((unsigned long long)(c + 1) << 4);
+        const unsigned long long* w = &a.ctl->decision[parity].v;
+        *sh_keep = word_wait_ge(w, want, a.ctl, fb)
+                       ? (int)(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0xFull)
+                       : -1;
+    }
+    __syncthreads();
+    const int keep = *sh_keep;
+    __syncthreads();
+    return keep;
+}
+
+// A whole warping without grid barriers.  Workgroups 0..nbw-1 iterate tiles
+// (workgroup b takes tiles b, b+nbw, ...); a tile starts chunk c as soon as
+// its neighbours have completed chunk c-1 (per-tile flags), so no workgroup
+// waits for the whole grid.  Workgroup nbw is the referee: it waits for the
+// error partials of chunk c from every workgroup, sums them in the fixed
+// order of the barrier kernel (grid_sums), applies the stopping rule and
+// publishes the decision.  Tiles do not wait for it: they iterate chunk c+1
+// from chunk c's output and check decision c before their first store of
+// chunk c+1.  "Continue": they store as usual.  "Stop after keep
+// iterations": they discard chunk c+1, and if keep is short of chunk c's
+// length they re-run keep iterations from chunk c's input, which no store of
+// chunk c+1 has touched.  Same arithmetic, partial sums, order and
+// decisions as k_tvl1_persist, so the same bits and iteration counts.
+template <int K, int R, int NW = 4, bool SQ = false>
+__global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist_nbr(IterArgs a, unsigned tiles_x, unsigned ntiles)
+{
+    static_assert(SQ ? (kSqTile - 2 * K >= K) : (NW * R - 2 * K >= K && 64 - 2 * K >= K),
+                  "the halo of a tile must lie within its neighbours");
+    __shared__ TileSharedN<NW> sh;
+    __shared__ SqShared sh_sq;
+    __shared__ double sh_err[NW][K];
+    __shared__ double sh_tot[K];
+    __shared__ double sh4[K * NW];
+    __shared__ int sh_keep;
+
+    TvCtl* ctl = a.ctl;
+    const unsigned nbw = gridDim.x - 1;   // tile workgroups; the last one is the referee
+    const unsigned tiles_y = ntiles / tiles_x;
+    int cur = ctl->cur[a.level];
+    int n = 0, c = 0, parity = 0;
+    // the first wait of a plain launch also checks that the grid is resident
+    FirstBound fb{a.resident_ticks, a.force_not_resident};
+
+    if (blockIdx.x == nbw) {   // referee
+        while (true) {
+            int kk = a.max_iter - n;
+            if (kk > K) kk = K;
+            if (kk <= 0) break;
+            if (threadIdx.x == 0) {
+                // every tile workgroup's first wait checks residency; the
+                // referee waits for chunk 0 of all of them without that bound
+                sh_keep = count_wait(&ctl->chunk_count[parity].v, nbw, ctl) ? 1 : 0;
+            }
+            __syncthreads();
+            if (!sh_keep) return;
+            const double* pp = a.partials + (size_t)parity * K * nbw;
+            grid_sums<K, NW>(pp, nbw, kk, sh4, sh_tot);
+            const int keep = stop_after(sh_tot, kk, n, a);
+            if (a.err_log && threadIdx.x < (unsigned)kk) a.err_log[n + threadIdx.x] = sh_tot[threadIdx.x];
+            if (threadIdx.x == 0) {
+                // the counter is reused two chunks on, by increments that
+                // follow this decision
+                __hip_atomic_store(&ctl->chunk_count[parity].v, 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __hip_atomic_store(&ctl->decision[parity].v,
+                                   (a.seq << 24) | ((unsigned long long)(c + 1) << 4) |
+                                       (unsigned long long)keep,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            n += keep ? keep : kk;
+            cur ^= 1;
+            parity ^= 1;
+            c++;
+            if (keep) break;
+        }
+        if (threadIdx.x == 0) {
+            ctl->cur[a.level] = cur;
+            ctl->n = n;
+            ctl->stop = 1;
+            *a.iters_slot = n;
+        }
+        return;
+    }
+
+    int kk_prev = 0;
+    int keep_prev = 0;
+    while (true) {
+        int kk = a.max_iter - n;
+        if (kk > K) kk = K;
+        if (kk <= 0) {   // past the last chunk: its decision says whether to re-run part of it
+            if (c > 0) keep_prev = wait_decision(a, c - 1, parity ^ 1, fb, &sh_keep);
+            break;
+        }
+        double acc[K];
+#pragma unroll
+        for (int t = 0; t < K; t++) acc[t] = 0.0;
+        bool decided = c == 0;
+        bool failed = false;
+        // decision c-1 before the chunk's first store: false discards
+        auto check = [&]() -> bool {
+            if (decided) return true;
+            decided = true;
+            keep_prev = wait_decision(a, c - 1, parity ^ 1, fb, &sh_keep);
+            fb = FirstBound{0ull, 0};
+            if (keep_prev < 0) failed = true;
+            return keep_prev == 0;
+        };
+        for (unsigned tile = blockIdx.x; tile < ntiles; tile += nbw) {
+            const int bx = (int)(tile % tiles_x), by = (int)(tile / tiles_x);
+            if (c > 0) {
+                if (!wait_neighbours(a, tiles_x, tiles_y, bx, by, c, fb)) return;
+                fb = FirstBound{0ull, 0};
+            }
+            if constexpr (SQ) tile_iterate_sq<K, true>(a, bx, by, cur, kk, acc, sh_sq, check);
+            else tile_iterate<K, R, true, NW>(a, bx, by, cur, kk, acc, sh, check);
+            if (failed) return;
+            if (keep_prev != 0) break;
+            // publish the tile's chunk once its stores have completed
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(a.tile_flags + tile, (a.seq << 20) | (unsigned)(c + 1),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (keep_prev != 0) break;
+        const double part = block_sum<K, NW>(acc, sh_err, threadIdx.x);
+        double* pp = a.partials + (size_t)parity * K * nbw;
+        if (threadIdx.x < (unsigned)kk)
+            __hip_atomic_store(pp + (size_t)threadIdx.x * nbw + blockIdx.x, part, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(&ctl->chunk_count[parity].v, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        kk_prev = kk;
+        n += kk;
+        cur ^= 1;
+        parity ^= 1;
+        c++;
+    }
+    if (keep_prev < 0) return;
+    // the warping stopped after keep_prev iterations of chunk c-1: re-run
+    // them from its input (buffer cur^1) into its output (buffer cur)
+    if (keep_prev > 0 && keep_prev < kk_prev) {
+        double acc[K];
+#pragma unroll
+        for (int t = 0; t < K; t++) acc[t] = 0.0;
+        for (unsigned tile = blockIdx.x; tile < ntiles; tile += nbw) {
+            const int bx = (int)(tile % tiles_x), by = (int)(tile / tiles_x);
+            if constexpr (SQ) tile_iterate_sq<K, true>(a, bx, by, cur ^ 1, keep_prev, acc, sh_sq);
+            else tile_iterate<K, R, true, NW>(a, bx, by, cur ^ 1, keep_prev, acc, sh);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // lockstep driver: one scale of many pairs per launch
 // ---------------------------------------------------------------------------
 
@@ -991,11 +1234,23 @@ int launch_iter(int K, int R, const IterArgs& a, hipStream_t s)
 
 // The persistent warping kernel of launch shape (K, R): 64 x 4R tiles as
 // kIterWaves waves, or (R = 0) square tiles as 4 waves.
-template <int K, int R>
+template <int K, int R, bool NBR>
 static const void* persist_kernel()
 {
-    if constexpr (R == 0) return (const void*)k_tvl1_persist<K, 0, 4, true>;
-    else return (const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>;
+    if constexpr (NBR) {
+        if constexpr (R == 0) return (const void*)k_tvl1_persist_nbr<K, 0, 4, true>;
+        else return (const void*)k_tvl1_persist_nbr<K, R * 4 / kIterWaves, kIterWaves>;
+    } else {
+        if constexpr (R == 0) return (const void*)k_tvl1_persist<K, 0, 4, true>;
+        else return (const void*)k_tvl1_persist<K, R * 4 / kIterWaves, kIterWaves>;
+    }
+}
+
+unsigned iter_tiles(int K, int R, int nx, int ny)
+{
+    unsigned gx, gy;
+    iter_grid(K, R, nx, ny, &gx, &gy);
+    return gx * gy;
 }
 
 template <int R>
@@ -1036,7 +1291,13 @@ void set_persist_host_paced(bool on) { g_host_paced.store(on ? 1 : 0); }
 
 void set_persist_force_not_resident(bool on) { g_force_not_resident.store(on ? 1 : 0); }
 
-template <int K, int R>
+static std::atomic<unsigned long long> g_warp_seq{0};
+
+unsigned long long next_warp_seq() { return g_warp_seq.fetch_add(1) + 1; }
+
+// NBR: the neighbour-synchronised kernel, whose grid is the tile
+// workgroups plus one referee workgroup.
+template <int K, int R, bool NBR>
 static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
 {
     static int resident = -1;   // co-resident workgroups of this kernel on the device
@@ -1046,7 +1307,7 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel<K, R>(),
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel<K, R, NBR>(),
                                                          persist_threads<R>(), 0) != hipSuccess) {
             (void)hipGetLastError();
             return OF_ERR_HIP;
@@ -1058,20 +1319,25 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
         // bound allows 2 of 8 waves)
         resident = std::min(per_cu, 4) * cus;
     }
-    if (resident <= 0) return OF_ERR_UNSUPPORTED;
+    if (resident <= (NBR ? 1 : 0)) return OF_ERR_UNSUPPORTED;
     unsigned gx, gy;
     iter_grid(K, R, a.nx, a.ny, &gx, &gy);
     const unsigned ntiles = gx * gy;
     unsigned nb = nblocks > 0 ? (unsigned)nblocks : ntiles;
     if (nb > ntiles) nb = ntiles;
-    if (nb > (unsigned)resident) nb = (unsigned)resident;
+    if (nb > (unsigned)resident - (NBR ? 1u : 0u)) nb = (unsigned)resident - (NBR ? 1u : 0u);
+    if (NBR) {
+        if (!a.tile_flags || a.seq == 0 || a.seq >= kNbrMaxSeq) return OF_ERR_ARG;
+        if ((long long)a.max_iter / K + 1 >= kNbrMaxChunks) return OF_ERR_UNSUPPORTED;
+        nb += 1;   // the referee
+    }
     IterArgs args = a;
     args.resident_ticks = bound_ticks;
     args.force_not_resident = g_force_not_resident.load(std::memory_order_relaxed);
     unsigned tx = gx, nt = ntiles;
     void* kargs[] = {(void*)&args, (void*)&tx, (void*)&nt};
     const hipError_t e =
-        hipLaunchKernel(persist_kernel<K, R>(), dim3(nb), dim3(persist_threads<R>()), kargs, 0, s);
+        hipLaunchKernel(persist_kernel<K, R, NBR>(), dim3(nb), dim3(persist_threads<R>()), kargs, 0, s);
     if (e != hipSuccess) {
         record_hip_error(e, "k_tvl1_persist launch", __FILE__, __LINE__);
         return OF_ERR_HIP;
@@ -1079,10 +1345,13 @@ static int launch_persist_kr(const IterArgs& a, int nblocks, hipStream_t s)
     return OF_OK;
 }
 
-int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s)
+int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s, int mode)
 {
-#define OF_PCASE(k, r) \
-    if (K == k && R == r) return launch_persist_kr<k, r>(a, nblocks, s);
+    if (mode != kPersistBarrier && mode != kPersistNeighbour) return OF_ERR_ARG;
+#define OF_PCASE(k, r)                                                            \
+    if (K == k && R == r)                                                         \
+        return mode == kPersistNeighbour ? launch_persist_kr<k, r, true>(a, nblocks, s) \
+                                         : launch_persist_kr<k, r, false>(a, nblocks, s);
     OF_PCASE(1, 0) OF_PCASE(2, 0) OF_PCASE(4, 0)
     OF_PCASE(1, 2) OF_PCASE(2, 2)
     OF_PCASE(1, 4) OF_PCASE(2, 4) OF_PCASE(4, 4)

@@ -29,6 +29,8 @@ __global__ void k_ctl_init(TvCtl* ctl)
         ctl->minkey[0] = ctl->minkey[1] = ~0ull;
         ctl->bar_count = 0;
         for (int g = 0; g < kBarGroups; g++) ctl->grp_count[g].v = 0;
+        ctl->chunk_count[0].v = ctl->chunk_count[1].v = 0;
+        ctl->decision[0].v = ctl->decision[1].v = 0ull;
         ctl->error = 0;
     }
 }
@@ -42,6 +44,7 @@ __global__ void k_ctl_begin(TvCtl* ctl)
         ctl->redo = 0;
         ctl->counter = 0;
         ctl->wlaunches = 0;
+        ctl->chunk_count[0].v = ctl->chunk_count[1].v = 0;
     }
 }
 
@@ -404,6 +407,7 @@ __global__ __launch_bounds__(256) void k_warp_setup(const float* __restrict__ I0
         ctl->redo = 0;
         ctl->counter = 0;
         ctl->wlaunches = 0;
+        ctl->chunk_count[0].v = ctl->chunk_count[1].v = 0;
     }
     if (j >= nx || i >= ny) return;
     warp_setup_px(I0, src, ctl->cur[level] ? ub1 : ub0, wx, wy, rhoc, nx, ny, i, j);

@@ -16,6 +16,9 @@ constexpr int kBarGroups = 8;
 struct alignas(256) BarWord {
     unsigned v;
 };
+struct alignas(256) SeqWord {
+    unsigned long long v;
+};
 
 // Device control block of one solve.  Written by the last workgroup of each
 // iteration launch (and by the set-up kernels), read by every workgroup at
@@ -38,6 +41,11 @@ struct TvCtl {
     alignas(256) unsigned bar_gen;
     BarWord grp_count[kBarGroups];   // two-level barrier: per-group arrivals
     BarWord grp_gen[kBarGroups];     // and per-group release epochs
+    // neighbour-synchronised warping kernel, by chunk parity: workgroups
+    // whose error partials of the chunk are published, and the stopping
+    // decision of the chunk, (seq << 24) | ((chunk + 1) << 4) | keep
+    BarWord chunk_count[2];
+    SeqWord decision[2];
     alignas(256) int error;         // kBarTimeout or kBarNotResident, else 0
 };
 
@@ -71,7 +79,24 @@ struct IterArgs {
     // for all workgroups before it reports kBarNotResident (0: no bound)
     unsigned long long resident_ticks;
     int force_not_resident;   // test hook: the first barrier reports kBarNotResident
+    // neighbour-synchronised warping kernel: per tile, (seq << 20) | chunks
+    // completed.  `seq` numbers the warpings of the process from 1 (never
+    // reused, below 2^40): it tags a warping's flags and decisions, so words
+    // left by earlier warpings, of any handle whose memory a buffer reuses,
+    // always compare below the current warping's
+    unsigned long long* tile_flags;
+    unsigned long long seq;
 };
+
+// Launch modes of a warping (LaunchRule::persistent, of_tvl1_set_launch_rule).
+constexpr int kHostPaced = 0;
+constexpr int kPersistBarrier = 1;     // one launch, grid barrier between chunks
+constexpr int kPersistNeighbour = 2;   // one launch, neighbour flags between chunks
+// Chunks of one neighbour-synchronised warping are counted in 20 bits.
+constexpr int kNbrMaxChunks = (1 << 20) - 2;
+constexpr unsigned long long kNbrMaxSeq = 1ull << 40;
+// The next warping number of the process (tvl1_iter.hip).
+unsigned long long next_warp_seq();
 
 // Launch helpers implemented in tvl1_iter.hip.
 int launch_iter(int K, int R, const IterArgs& a, hipStream_t s);
@@ -80,7 +105,11 @@ void iter_grid(int K, int R, int nx, int ny, unsigned* gx, unsigned* gy);
 // Whole warping in one launch of a grid that fits the device at once;
 // `nblocks` = 0 picks the largest such grid not above the tile count.  Its
 // first grid barrier reports kBarNotResident if the grid did not fully start.
-int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s);
+// `mode`: kPersistBarrier or kPersistNeighbour.
+int launch_persist(int K, int R, const IterArgs& a, int nblocks, hipStream_t s,
+                   int mode = kPersistBarrier);
+// Tiles of launch shape (K, R) at nx x ny (the size of a tile-flag array).
+unsigned iter_tiles(int K, int R, int nx, int ny);
 // Process-wide: true = warpings whose launch rule is persistent run
 // host-paced instead (after a residency failure, with AMDFLOW_HOST_PACED=1,
 // or when set); false (default) = persistent launches.
