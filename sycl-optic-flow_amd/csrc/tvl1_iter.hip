@@ -968,26 +968,31 @@ __global__ OF_ITER_BOUNDS(NW) void k_tvl1_persist_nbr(IterArgs a, unsigned tiles
             else tile_iterate<K, R, true, NW>(a, bx, by, cur, kk, acc, sh, check);
             if (failed) return;
             if (keep_prev != 0) break;
-            // publish the tile's chunk once its stores have completed
+            // after the workgroup's last tile of the chunk, its error partials
+            // go out with the tile's stores, so one wait for completion
+            // covers both
+            const bool last = tile + nbw >= ntiles;
+            if (last) {
+                const double part = block_sum<K, NW>(acc, sh_err, threadIdx.x);
+                double* pp = a.partials + (size_t)parity * K * nbw;
+                if (threadIdx.x < (unsigned)kk)
+                    __hip_atomic_store(pp + (size_t)threadIdx.x * nbw + blockIdx.x, part,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            // publish the tile's chunk (and the partials) once the stores
+            // have completed
             __builtin_amdgcn_s_waitcnt(0);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             __syncthreads();
-            if (threadIdx.x == 0)
+            if (threadIdx.x == 0) {
                 __hip_atomic_store(a.tile_flags + tile, (a.seq << 20) | (unsigned)(c + 1),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (last)
+                    __hip_atomic_fetch_add(&ctl->chunk_count[parity].v, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         if (keep_prev != 0) break;
-        const double part = block_sum<K, NW>(acc, sh_err, threadIdx.x);
-        double* pp = a.partials + (size_t)parity * K * nbw;
-        if (threadIdx.x < (unsigned)kk)
-            __hip_atomic_store(pp + (size_t)threadIdx.x * nbw + blockIdx.x, part, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __syncthreads();
-        if (threadIdx.x == 0)
-            __hip_atomic_fetch_add(&ctl->chunk_count[parity].v, 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
         kk_prev = kk;
         n += kk;
         cur ^= 1;
