@@ -441,11 +441,15 @@ constexpr int kIterBlocksPerCU = 3;
 // which hides the latency of an iteration better than 4 waves of R rows:
 // host-paced, every shape was faster at every scale of a 1920x1080 pyramid
 // (e.g. (4,4) at 480x270: 5.22 -> 4.87 us per iteration; (4,8) at 960x540:
-// 9.29 -> 8.18 us).  Compiled for 2 workgroups per CU (128 VGPRs).
+// 9.29 -> 8.18 us).  OF_ITER_BOUNDS asks for at least 2 waves per SIMD
+// (its second argument counts waves per SIMD, not workgroups per CU); the
+// 8-wave kernels hold 2 workgroups per CU wherever they stay within 128
+// VGPRs, which all shapes but (8,8) do.
 constexpr int kIterWaves = 8;
 // The lockstep batch kernel keeps 4 waves of R rows: with 8 waves of R/2 the
 // batch of 64 mixed-size pairs took 79.0 ms instead of 75.7 ms (same box).
 constexpr int kLsWaves = 4;
+// __launch_bounds__(threads, minimum waves per SIMD)
 #define OF_ITER_BOUNDS(NW) __launch_bounds__(64 * (NW), (NW) == 4 ? kIterBlocksPerCU : 2)
 
 // ---------------------------------------------------------------------------
