@@ -257,9 +257,13 @@ __device__ __forceinline__ void tile_iterate(const IterArgs& a, int bx, int by, 
 // ten thousand pixels (120x68: 1.2 instead of 3.3 us per iteration).
 constexpr int kSqTile = 16;
 
+// p exchange in a..d, u exchange in e, f: the two halves of an iteration
+// use different arrays, so each needs one barrier, between its writes and
+// its reads.  A half's next writes come after the other half's barrier,
+// which every thread reaches only after its reads of this half.
 struct SqShared {
     float a[kSqTile][kSqTile + 1], b[kSqTile][kSqTile + 1], c[kSqTile][kSqTile + 1],
-        d[kSqTile][kSqTile + 1];
+        d[kSqTile][kSqTile + 1], e[kSqTile][kSqTile + 1], f[kSqTile][kSqTile + 1];
 };
 
 // kk (<= K) iterations on square tile (bx, by): reads buffer `cur`, writes
@@ -312,7 +316,6 @@ __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int b
             __syncthreads();
             const float p11l = sh.a[ly][xm], p21l = sh.b[ly][xm];
             const float p12u = sh.c[ym][lx], p22u = sh.d[ym][lx];
-            __syncthreads();
             const float div1 = div_rule(p11, p11l, p12, p12u, fr, lr, fc, lc);
             const float div2 = div_rule(p21, p21l, p22, p22u, fr, lr, fc, lc);
             float v1, v2;
@@ -325,12 +328,11 @@ __device__ __forceinline__ void tile_iterate_sq(const IterArgs& a, int bx, int b
             u1 = nu1;
             u2 = nu2;
             // ---- p update: forward gradient of u, g, p = (p + taut*grad)/g ----
-            sh.a[ly][lx] = u1;
-            sh.b[ly][lx] = u2;
+            sh.e[ly][lx] = u1;
+            sh.f[ly][lx] = u2;
             __syncthreads();
-            const float u1r = sh.a[ly][xp], u2r = sh.b[ly][xp];
-            const float u1d = sh.a[yp][lx], u2d = sh.b[yp][lx];
-            __syncthreads();
+            const float u1r = sh.e[ly][xp], u2r = sh.f[ly][xp];
+            const float u1d = sh.e[yp][lx], u2d = sh.f[yp][lx];
             const float fx1 = xl ? u1r - u1 : 0.0f;
             const float fy1 = yl ? u1d - u1 : 0.0f;
             const float fx2 = xl ? u2r - u2 : 0.0f;
