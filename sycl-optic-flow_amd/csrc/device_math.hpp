@@ -22,13 +22,17 @@ __device__ __forceinline__ int neumann_clamp(int x, int n, bool& out)
 // bicubic_interpolation.cpp:38-46.  The literals are double in the
 // reference, the two bare differences are float, the result is rounded to
 // float.
+// The products of a float by 3, 5 and 4 are exact in double, so each
+// product-then-sum below is one fused multiply-add with the same rounding
+// (and the same infinities, NaNs and signed zeros).
 __device__ __forceinline__ float cubic_cell(float v0, float v1, float v2, float v3, float x)
 {
     const double xd = (double)x;
     const float d20 = v2 - v0;
     const float d12 = v1 - v2;
-    const double c3 = ((3.0 * (double)d12) + (double)v3) - (double)v0;
-    const double c2 = (((2.0 * (double)v0) - (5.0 * (double)v1)) + (4.0 * (double)v2)) - (double)v3;
+    const double c3 = __builtin_fma(3.0, (double)d12, (double)v3) - (double)v0;
+    const double c2 = __builtin_fma(4.0, (double)v2, __builtin_fma(-5.0, (double)v1, 2.0 * (double)v0)) -
+                      (double)v3;
     const double inner = (double)d20 + xd * (c2 + xd * c3);
     return (float)((double)v1 + (0.5 * xd) * inner);
 }

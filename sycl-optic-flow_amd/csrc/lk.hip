@@ -246,16 +246,15 @@ __global__ __launch_bounds__(256) void k_lk_tensor_fused(const float* __restrict
     __syncthreads();
     for (int e = threadIdx.x; e < TW * EH; e += 256) {
         const int ey = e / TW, ox = e - ey * TW;
-        const int gx = x0 + ox;
+        // products outside the image are +0.0, and adding +0.0 leaves a
+        // sum that starts at +0.0 unchanged: every tap is added, with the
+        // bits of the image-clipped window sum
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
         for (int k = -WC; k <= WC; k++) {
-            const int jj = gx + k;
-            if (jj >= 0 && jj < w) {
-                s0 += pr[0][ey][ox + WC + k];
-                s1 += pr[1][ey][ox + WC + k];
-                s2 += pr[2][ey][ox + WC + k];
-            }
+            s0 += pr[0][ey][ox + WC + k];
+            s1 += pr[1][ey][ox + WC + k];
+            s2 += pr[2][ey][ox + WC + k];
         }
         rs[0][ey][ox] = s0;
         rs[1][ey][ox] = s1;
@@ -271,12 +270,9 @@ __global__ __launch_bounds__(256) void k_lk_tensor_fused(const float* __restrict
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
         for (int kk = -WC; kk <= WC; kk++) {
-            const int ii = gy + kk;
-            if (ii >= 0 && ii < h) {
-                s0 += rs[0][oy + WC + kk][ox];
-                s1 += rs[1][oy + WC + kk][ox];
-                s2 += rs[2][oy + WC + kk][ox];
-            }
+            s0 += rs[0][oy + WC + kk][ox];
+            s1 += rs[1][oy + WC + kk][ox];
+            s2 += rs[2][oy + WC + kk][ox];
         }
         const size_t p = (size_t)gy * w + gx;
         G[p] = s0;
@@ -373,6 +369,35 @@ __global__ __launch_bounds__(256) void k_lk_update(const float* __restrict__ row
 // of it.
 constexpr int kLkBlockEdge = 64;
 
+// Whole-wave lane shifts for the row sums: lane i gets lane i-1's (from the
+// left) or lane i+1's value; the lane without a source gets 0.  Those lanes
+// (0 and 63) lie outside every square the blocked kernel stores.
+__device__ __forceinline__ float lane_from_left0(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138 /* wave_shr:1 */, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float lane_from_right0(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130 /* wave_shl:1 */, 0xf, 0xf, true));
+}
+
+// clampi(f2i_sat(f), -1, n) for an integer-valued f (a floor) and 0 < n <
+// 2^24, without branches: NaN gives 0, everything else is clamped in float
+// (where -1 and n are exact) and converted.
+__device__ __forceinline__ int floor_index(float f, int n)
+{
+    const int c = (int)fminf(fmaxf(f, -1.0f), (float)n);
+    return f != f ? 0 : c;
+}
+
+// img[idx] through a 32-bit byte offset (the blocked kernel runs on planes
+// of at most 2^30 floats).
+__device__ __forceinline__ float load_at(const float* __restrict__ img, int idx)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(img) + (unsigned)idx * 4u);
+}
+
 template <int WC>
 __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restrict__ P0,
                                                            const float* __restrict__ P1,
@@ -395,25 +420,24 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
     const size_t n = (size_t)w * h;
     const int gx = x0 + ex;
     const bool xin = gx >= 0 && gx < w;
-    bool cin[2 * WC + 1];   // column gx + q inside the image
-#pragma unroll
-    for (int q = -WC; q <= WC; q++) cin[q + WC] = gx + q >= 0 && gx + q < w;
 
     float uu[NP], vv[NP], i0[NP], gxv[NP], gyv[NP], gxx[NP], gxy[NP], gyy[NP];
     bool in[NP];
+    const float* __restrict__ Gxy = G + n;
+    const float* __restrict__ Gyy = G + 2 * n;
 #pragma unroll
     for (int k = 0; k < NP; k++) {
         const int gy = y0 + ey0 + k;
         in[k] = xin && gy >= 0 && gy < h;
-        const size_t p = in[k] ? (size_t)gy * w + gx : 0;
-        uu[k] = in[k] ? u[p] : 0.0f;
-        vv[k] = in[k] ? v[p] : 0.0f;
-        i0[k] = P0[p];
-        gxv[k] = Ix[p];
-        gyv[k] = Iy[p];
-        gxx[k] = G[p];
-        gxy[k] = G[n + p];
-        gyy[k] = G[2 * n + p];
+        const int p = in[k] ? gy * w + gx : 0;
+        uu[k] = in[k] ? load_at(u, p) : 0.0f;
+        vv[k] = in[k] ? load_at(v, p) : 0.0f;
+        i0[k] = load_at(P0, p);
+        gxv[k] = load_at(Ix, p);
+        gyv[k] = load_at(Iy, p);
+        gxx[k] = load_at(G, p);
+        gxy[k] = load_at(Gxy, p);
+        gyy[k] = load_at(Gyy, p);
     }
     for (int it = 0; it < kk; it++) {
         const int lo1 = (it + 1) * WC, hi1 = E - (it + 1) * WC;   // flow exact here after it
@@ -430,13 +454,15 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
             const float fx0 = floorf(x), fy0 = floorf(y);
             ax[k] = x - fx0;
             ay[k] = y - fy0;
-            const int xi = clampi(f2i_sat(fx0), -1, w), yi = clampi(f2i_sat(fy0), -1, h);
+            const int xi = floor_index(fx0, w), yi = floor_index(fy0, h);
             const int xa = clampi(xi, 0, w - 1), xb = clampi(xi + 1, 0, w - 1);
             const int ya = clampi(yi, 0, h - 1), yb = clampi(yi + 1, 0, h - 1);
-            ta[k] = P1[(size_t)ya * w + xa];
-            tb[k] = P1[(size_t)ya * w + xb];
-            tc[k] = P1[(size_t)yb * w + xa];
-            td[k] = P1[(size_t)yb * w + xb];
+            const int ra = (int)__umul24((unsigned)ya, (unsigned)w);
+            const int rb = (int)__umul24((unsigned)yb, (unsigned)w);
+            ta[k] = load_at(P1, ra + xa);
+            tb[k] = load_at(P1, ra + xb);
+            tc[k] = load_at(P1, rb + xa);
+            td[k] = load_at(P1, rb + xb);
         }
         float r0[NP], r1[NP];   // row sums of this thread's rows
 #pragma unroll
@@ -450,18 +476,20 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
             c[WC] = in[k] ? gyv[k] * itv : 0.0f;
 #pragma unroll
             for (int d = 1; d <= WC; d++) {
-                a[WC + d] = from_right(a[WC + d - 1]);
-                c[WC + d] = from_right(c[WC + d - 1]);
-                a[WC - d] = from_left(a[WC - d + 1]);
-                c[WC - d] = from_left(c[WC - d + 1]);
+                a[WC + d] = lane_from_right0(a[WC + d - 1]);
+                c[WC + d] = lane_from_right0(c[WC + d - 1]);
+                a[WC - d] = lane_from_left0(a[WC - d + 1]);
+                c[WC - d] = lane_from_left0(c[WC - d + 1]);
             }
+            // every tap is added: a tap outside the image carries +0.0 (its
+            // lane's `in` is false), and s + (+0.0) is s for every s but
+            // -0.0, which a sum that starts at +0.0 never is; so the sum has
+            // the bits of the image-clipped window sum
             float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
             for (int q = 0; q <= 2 * WC; q++) {
-                if (cin[q]) {
-                    s0 += a[q];
-                    s1 += c[q];
-                }
+                s0 += a[q];
+                s1 += c[q];
             }
             r0[k] = s0;
             r1[k] = s1;
@@ -487,18 +515,17 @@ __global__ __launch_bounds__(1024) void k_lk_iters_blocked(const float* __restri
             y0s[WC + k] = r0[k];
             y1s[WC + k] = r1[k];
         }
+        // rows outside the image hold row sums of +0.0, so the column sums
+        // need no clipping either
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-            const int ey = ey0 + k, gy = y0 + ey;
+            const int ey = ey0 + k;
             if (!in[k] || ey < lo1 || ey >= hi1 || ex < lo1 || ex >= hi1) continue;
             float bx = 0.0f, by = 0.0f;
 #pragma unroll
             for (int q = -WC; q <= WC; q++) {
-                const int ii = gy + q;
-                if (ii >= 0 && ii < h) {
-                    bx += y0s[WC + k + q];
-                    by += y1s[WC + k + q];
-                }
+                bx += y0s[WC + k + q];
+                by += y1s[WC + k + q];
             }
             const float det = gxx[k] * gyy[k] - gxy[k] * gxy[k];
             if (det > det_eps) {
@@ -731,7 +758,11 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
         const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b, n = len * sizeof(float);
         return x < y + n && y < x + n;
     };
-    const bool direct = wc >= 1 && wc <= 4 && iters >= 1 && !overlaps(du, dv, n0) &&
+    // the blocked iterations index planes with 32-bit byte offsets and
+    // 24-bit row products; level 0 is the largest level
+    const bool blocked = wc >= 1 && wc <= 4 && nx[0] < (1 << 24) && ny[0] < (1 << 24) &&
+                         n0 <= ((size_t)1 << 30);
+    const bool direct = blocked && iters >= 1 && !overlaps(du, dv, n0) &&
                         !overlaps(du, dI0, n0) && !overlaps(du, dI1, n0) &&
                         !overlaps(dv, dI0, n0) && !overlaps(dv, dI1, n0);
     const float invZ = 1 / zf;
@@ -764,7 +795,6 @@ int lk_pyr_run(const LkPlan& P, const LkBufs& B, const float* dI0, const float* 
             LK_CHECK("k_lk_boxsum");
         }
         // windows up to 9 x 9: blocks of iterations per launch (k_lk_iters_blocked)
-        const bool blocked = wc >= 1 && wc <= 4;
         for (int it = 0; it < iters && blocked;) {
             const int kb = lk_block_iters(wc, (long long)w * h);
             const int kk = iters - it < kb ? iters - it : kb;

@@ -184,81 +184,84 @@ __global__ __launch_bounds__(256) void k_zoom_out_sample(const float* __restrict
     dst[(size_t)i1 * nxx + j1] = zoom_out_px(src, nx, ny, i1, j1, factor);
 }
 
-// zoom_in of both flow components (zoom.cpp:61-85) fused with the
-// cblas_sscal by 1/zfactor (tvl1.cpp:171-172).  Reads the coarse scale's
-// current buffer, writes buffer 0 of the finer scale.
-// Tiled separable Gaussian of two frames per launch (blockIdx.z = frame),
-// the row pass kept in LDS.  Output tile: 64 columns x TR rows of an
-// output image whose pixel (oj, oi) is the blurred input at (STEP*oj,
-// STEP*oi):
+// Tiled separable Gaussian of two frames per launch (blockIdx.z = frame).
+// Output tile: 64 columns x TR rows of an output image whose pixel (oj, oi)
+// is the blurred input at (STEP*oj, STEP*oi):
 //   STEP 1: the blurred image (the pre-smoothing, tvl1.cpp:143-144); with
 //           NORM the input is first normalised as image_normalization does
 //           (tvl1.cpp:302-329), sample by sample;
 //   STEP 2: zoom_out by 1/2 (zoom.cpp:20-53): the bicubic at an integer
 //           position returns the sample there, so the kept samples are the
 //           blurred image at even positions and only those are blurred.
-// The row pass runs at the STEP*oj columns of every row the column pass
-// reads (rows outside the image hold the mirrored row's values, as
-// mask.cpp:240-242, 261-263 read them), then the column pass at the
-// STEP*oi rows: the per-pixel arithmetic of k_gauss_rows, k_gauss_cols and
-// k_zoom_out_sample, so the same values (an exact zero may differ in sign
-// from the bicubic's, which only matters for inputs holding -0).  Needs
-// taps of at most kBlurHalo + 1.
+// Every source sample the tile reads (rows and columns outside the image
+// hold the mirrored sample, as mask.cpp:240-242, 261-263 read them) is
+// staged in LDS first, all loads of a thread issued before the first is
+// stored; then the row pass runs at the STEP*oj columns of every staged
+// row, and the column pass at the STEP*oi rows: the per-pixel arithmetic of
+// k_gauss_rows, k_gauss_cols and k_zoom_out_sample, so the same values (an
+// exact zero may differ in sign from the bicubic's, which only matters for
+// inputs holding -0).  H is the largest half-width the instance stages
+// (taps of at most H + 1).
 constexpr int kBlurHalo = 15;
 
-template <int TR, int STEP, bool NORM>
+template <int TR, int STEP, bool NORM, int H>
 __global__ __launch_bounds__(256) void k_blur_tile(const float* __restrict__ a,
                                                    const float* __restrict__ b,
                                                    float* __restrict__ oa, float* __restrict__ ob,
                                                    int nx, int ny, int onx, int ony, GaussTaps g,
                                                    const TvCtl* ctl)
 {
-    constexpr int LR = STEP * (TR - 1) + 1 + 2 * kBlurHalo;   // LDS rows
-    constexpr int SW = 64 + 2 * kBlurHalo;                     // staged columns (NORM)
+    constexpr int LR = STEP * (TR - 1) + 1 + 2 * H;    // staged rows
+    constexpr int SW = STEP * 63 + 1 + 2 * H;          // staged columns
+    constexpr int KR = (LR + 3) / 4, KC = (SW + 63) / 64;
+    __shared__ float S[LR][SW];
     __shared__ float R[LR][64];
-    __shared__ float S[NORM ? LR : 1][NORM ? SW : 1];
     const float* src = blockIdx.z ? b : a;
     float* dst = blockIdx.z ? ob : oa;
     const int oj0 = blockIdx.x * 64, oi0 = blockIdx.y * TR;
     const int hw = g.size - 1;
     const int oilast = min(oi0 + TR, ony) - 1;
-    const int rlo = STEP * oi0 - hw, rhi = STEP * oilast + hw;
-    const int nrows = rhi - rlo + 1;
-    if constexpr (NORM) {
-        // the source window normalised once per sample (rows and columns
-        // outside the image hold the mirrored sample, as the blur reads it)
-        float mn, den;
-        norm_bounds(ctl, a, b, &mn, &den);
-        const int xlo = oj0 - hw, ncols = min(oj0 + 64, onx) - 1 + hw - xlo + 1;
-        for (int e = threadIdx.x; e < nrows * SW; e += 256) {
-            const int lr = e / SW, c = e - lr * SW;
-            if (c >= ncols) continue;
-            const int r = rlo + lr, x = xlo + c;
-            const int y = r < 0 ? -r : (r >= ny ? 2 * ny - 1 - r : r);
-            const int xx = x < 0 ? -x : (x >= nx ? 2 * nx - 1 - x : x);
-            S[lr][c] = norm_px(src[(size_t)y * nx + xx], mn, den);
+    const int ojlast = min(oj0 + 64, onx) - 1;
+    const int rlo = STEP * oi0 - hw, nrows = STEP * (oilast - oi0) + 1 + 2 * hw;
+    const int xlo = STEP * oj0 - hw, ncols = STEP * (ojlast - oj0) + 1 + 2 * hw;
+    // thread t stages rows 4*kr + t/64 and columns 64*kc + t%64; a slot
+    // beyond the window loads its clamped neighbour and stores nothing
+    const int tr = (int)threadIdx.x >> 6, tc = (int)threadIdx.x & 63;
+    float v[KR][KC];
+#pragma unroll
+    for (int kr = 0; kr < KR; kr++) {
+        const int lr = min(4 * kr + tr, nrows - 1);
+        const int r = rlo + lr;
+        const int y = r < 0 ? -r : (r >= ny ? 2 * ny - 1 - r : r);
+        const float* row = src + (size_t)y * nx;
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            const int x = xlo + min(64 * kc + tc, ncols - 1);
+            v[kr][kc] = row[x < 0 ? -x : (x >= nx ? 2 * nx - 1 - x : x)];
         }
-        __syncthreads();
     }
+    float mn = 0.0f, den = 0.0f;
+    if constexpr (NORM) norm_bounds(ctl, a, b, &mn, &den);
+#pragma unroll
+    for (int kr = 0; kr < KR; kr++) {
+        const int lr = 4 * kr + tr;
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            const int c = 64 * kc + tc;
+            if (lr < nrows && c < ncols) S[lr][c] = NORM ? norm_px(v[kr][kc], mn, den) : v[kr][kc];
+        }
+    }
+    __syncthreads();
     for (int e = threadIdx.x; e < nrows * 64; e += 256) {
         const int lr = e >> 6, c = e & 63;
-        const int oj = oj0 + c;
-        if (oj >= onx) continue;
-        float v;
-        if constexpr (NORM) {
-            const int sc = c + hw;   // column oj in S
-            double sum = g.B[0] * (double)S[lr][sc];
-            for (int j = 1; j < g.size; j++) {
-                const float pair = S[lr][sc - j] + S[lr][sc + j];
-                sum += g.B[j] * (double)pair;
-            }
-            v = (float)sum;
-        } else {
-            const int r = rlo + lr;
-            const int y = r < 0 ? -r : (r >= ny ? 2 * ny - 1 - r : r);
-            v = gauss_row_px(src, nx, STEP * oj, y, g);
+        if (oj0 + c > ojlast) continue;
+        const int sc = STEP * c + hw;   // source column STEP*oj in S
+        double sum = g.B[0] * (double)S[lr][sc];
+        for (int j = 1; j < g.size; j++) {
+            const float pair = S[lr][sc - j] + S[lr][sc + j];
+            sum += g.B[j] * (double)pair;
         }
-        R[lr][c] = v;
+        R[lr][c] = (float)sum;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < TR * 64; e += 256) {
@@ -276,29 +279,44 @@ __global__ __launch_bounds__(256) void k_blur_tile(const float* __restrict__ a,
 }
 
 // Rows per tile: 16, or fewer when the output image is small, so that the
-// launch still has a few hundred workgroups.
-template <int STEP, bool NORM>
-static void launch_blur_tile(const float* a, const float* b, float* oa, float* ob, int nx, int ny,
-                             int onx, int ony, const GaussTaps& g, const TvCtl* ctl, hipStream_t s)
+// launch still has a few hundred workgroups; the staged half-width 7 when
+// the taps allow it (the default TV-L1 and LK sigmas), else kBlurHalo.
+template <int STEP, bool NORM, int H>
+static void launch_blur_tile_h(const float* a, const float* b, float* oa, float* ob, int nx,
+                               int ny, int onx, int ony, const GaussTaps& g, const TvCtl* ctl,
+                               hipStream_t s)
 {
     const unsigned gx = cdiv(onx, 64);
     int tr = 16;
     while (tr > 2 && (size_t)gx * cdiv(ony, tr) * 2 < 512) tr /= 2;
     const dim3 grid(gx, cdiv(ony, tr), 2);
     if (tr == 16)
-        hipLaunchKernelGGL((k_blur_tile<16, STEP, NORM>), grid, dim3(256), 0, s, a, b, oa, ob, nx, ny,
-                           onx, ony, g, ctl);
+        hipLaunchKernelGGL((k_blur_tile<16, STEP, NORM, H>), grid, dim3(256), 0, s, a, b, oa, ob,
+                           nx, ny, onx, ony, g, ctl);
     else if (tr == 8)
-        hipLaunchKernelGGL((k_blur_tile<8, STEP, NORM>), grid, dim3(256), 0, s, a, b, oa, ob, nx, ny,
-                           onx, ony, g, ctl);
+        hipLaunchKernelGGL((k_blur_tile<8, STEP, NORM, H>), grid, dim3(256), 0, s, a, b, oa, ob,
+                           nx, ny, onx, ony, g, ctl);
     else if (tr == 4)
-        hipLaunchKernelGGL((k_blur_tile<4, STEP, NORM>), grid, dim3(256), 0, s, a, b, oa, ob, nx, ny,
-                           onx, ony, g, ctl);
+        hipLaunchKernelGGL((k_blur_tile<4, STEP, NORM, H>), grid, dim3(256), 0, s, a, b, oa, ob,
+                           nx, ny, onx, ony, g, ctl);
     else
-        hipLaunchKernelGGL((k_blur_tile<2, STEP, NORM>), grid, dim3(256), 0, s, a, b, oa, ob, nx, ny,
-                           onx, ony, g, ctl);
+        hipLaunchKernelGGL((k_blur_tile<2, STEP, NORM, H>), grid, dim3(256), 0, s, a, b, oa, ob,
+                           nx, ny, onx, ony, g, ctl);
 }
 
+template <int STEP, bool NORM>
+static void launch_blur_tile(const float* a, const float* b, float* oa, float* ob, int nx, int ny,
+                             int onx, int ony, const GaussTaps& g, const TvCtl* ctl, hipStream_t s)
+{
+    if (g.size - 1 <= 7)
+        launch_blur_tile_h<STEP, NORM, 7>(a, b, oa, ob, nx, ny, onx, ony, g, ctl, s);
+    else
+        launch_blur_tile_h<STEP, NORM, kBlurHalo>(a, b, oa, ob, nx, ny, onx, ony, g, ctl, s);
+}
+
+// zoom_in of both flow components (zoom.cpp:61-85) fused with the
+// cblas_sscal by 1/zfactor (tvl1.cpp:171-172).  Reads the coarse scale's
+// current buffer, writes buffer 0 of the finer scale.
 __global__ __launch_bounds__(256) void k_zoom_in_flow(const float* __restrict__ uc0,
                                                       const float* __restrict__ uc1,
                                                       float* __restrict__ uf, int nx, int ny,
