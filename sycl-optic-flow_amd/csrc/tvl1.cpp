@@ -342,8 +342,7 @@ int solve_levels_once(of_tvl1* h, const float* init, int first, const int* warps
     }
     h->run_first = first;
     // every scale's current buffer back to 0 (the pyramids stay as prepared)
-    OF_TRY(launch_ctl_init(h->ctl, st));
-    OF_HIP(hipMemsetAsync(h->d_iters, 0, sizeof(int) * h->h_iters.size(), st));
+    OF_TRY(launch_ctl_init(h->ctl, st, h->d_iters, (int)h->h_iters.size()));
     {
         const size_t n2 = 2 * (size_t)h->nx[first] * h->ny[first];
         float* uc = h->ubase + 4 * h->off[first];   // buffer 0
@@ -361,8 +360,8 @@ int solve_levels_once(of_tvl1* h, const float* init, int first, const int* warps
         float* I1s = h->I1p + h->off[s];
         float* ub = h->ubase + 4 * h->off[s];
         if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s], st));
-        OF_TRY(launch_centered_gradient_packed(I1s, h->wsrc, nx, ny, st));
-        OF_TRY(launch_fill(h->pbase, 4 * N, 0.0f, st));   // p = 0 in buffer 0, tvl1.cpp:190-193
+        // the packed gradient of I1, and p = 0 in buffer 0 (tvl1.cpp:190-193)
+        OF_TRY(launch_centered_gradient_packed(I1s, h->wsrc, nx, ny, st, h->pbase));
         for (int w = 0; w < warps; w++) {
             OF_TRY(launch_warp_setup(I0s, h->wsrc, ub, ub + 2 * N, h->wx, h->wy, h->rhoc, nx, ny,
                                      h->ctl, s, st));

@@ -15,8 +15,11 @@ namespace amdflow {
 // ---------------------------------------------------------------------------
 
 // Start of a solve: every scale's current buffer is 0, min/max keys reset.
-__global__ void k_ctl_init(TvCtl* ctl)
+// Also clears `iters[0..n_iters)` (the per-warping iteration counts), when
+// given.
+__global__ void k_ctl_init(TvCtl* ctl, int* iters, int n_iters)
 {
+    for (int k = threadIdx.x; k < n_iters; k += blockDim.x) iters[k] = 0;
     if (threadIdx.x < kMaxScales) ctl->cur[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         ctl->n = 0;
@@ -370,14 +373,17 @@ __global__ __launch_bounds__(256) void k_centered_gradient(const float* __restri
 
 // The centred gradient of I1 stored with I1 itself as one float4 per pixel
 // (I1, I1x, I1y, 0): the source of the warping's three bicubic samples.
+// With `pzero`, also clears the dual variables of the scale (4 floats per
+// pixel, tvl1.cpp:190-193).
 __global__ __launch_bounds__(256) void k_centered_gradient_packed(const float* __restrict__ in,
                                                                   float4* __restrict__ out, int nx,
-                                                                  int ny)
+                                                                  int ny, float4* __restrict__ pzero)
 {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j >= nx || i >= ny) return;
     gradient_packed_px(in, out, nx, ny, i, j);
+    if (pzero) pzero[(size_t)i * nx + j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_pack3(const float* __restrict__ a, const float* __restrict__ b,
@@ -539,9 +545,9 @@ static inline unsigned grid1d(size_t n) { unsigned g = cdiv(n, 256); return g > 
         }                                                           \
     } while (0)
 
-int launch_ctl_init(TvCtl* ctl, hipStream_t s)
+int launch_ctl_init(TvCtl* ctl, hipStream_t s, int* iters, int n_iters)
 {
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl);
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, iters, n_iters);
     OF_LAUNCH_CHECK("k_ctl_init");
     return OF_OK;
 }
@@ -666,10 +672,11 @@ int launch_fill(float* p, size_t n, float v, hipStream_t s)
     return OF_OK;
 }
 
-int launch_centered_gradient_packed(const float* in, float* out4, int nx, int ny, hipStream_t s)
+int launch_centered_gradient_packed(const float* in, float* out4, int nx, int ny, hipStream_t s,
+                                    float* pzero)
 {
     hipLaunchKernelGGL(k_centered_gradient_packed, grid2d(nx, ny), dim3(256), 0, s, in,
-                       reinterpret_cast<float4*>(out4), nx, ny);
+                       reinterpret_cast<float4*>(out4), nx, ny, reinterpret_cast<float4*>(pzero));
     OF_LAUNCH_CHECK("k_centered_gradient_packed");
     return OF_OK;
 }

@@ -9,7 +9,8 @@
 
 namespace amdflow {
 
-int launch_ctl_init(TvCtl* ctl, hipStream_t s);
+// Resets the control block; with `iters`, also clears n_iters counts.
+int launch_ctl_init(TvCtl* ctl, hipStream_t s, int* iters = nullptr, int n_iters = 0);
 int launch_ctl_begin(TvCtl* ctl, hipStream_t s);
 int launch_load_minmax_u8(const uint8_t* a, const uint8_t* b, float* fa, float* fb, size_t n,
                           TvCtl* ctl, hipStream_t s);
@@ -40,7 +41,9 @@ int launch_centered_gradient(const float* in, float* dx, float* dy, int nx, int 
 int launch_fill(float* p, size_t n, float v, hipStream_t s);
 int launch_scale(float* p, size_t n, float f, hipStream_t s);   // p *= f
 // (I1, I1x, I1y, 0) per pixel, 4 floats, 16-byte aligned
-int launch_centered_gradient_packed(const float* in, float* out4, int nx, int ny, hipStream_t s);
+// With `pzero` (16-byte aligned), also clears 4 floats per pixel there.
+int launch_centered_gradient_packed(const float* in, float* out4, int nx, int ny, hipStream_t s,
+                                    float* pzero = nullptr);
 int launch_pack3(const float* a, const float* b, const float* c, float* out4, size_t n, hipStream_t s);
 int launch_warp_setup(const float* I0, const float* src4, const float* ub0, const float* ub1,
                       float* wx, float* wy, float* rhoc, int nx, int ny, TvCtl* ctl, int level,
