@@ -453,7 +453,11 @@ def cpu_baseline(q0, q1, budget_s: float):
                      f"OpenMP {cores} threads",
            "single_thread": {"value": round(1.0 / med1, 4), "unit": "frame-pairs/s",
                              "sample": f"median of {runs1} solves, 1 thread"},
-           "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+           "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+           "note": "oracle port of the reference's sequential solver (the reference itself "
+                   "needs MKL and is not built here); its bicubic sampler, zoom_in and "
+                   "zoom_size equal reference-built vectors bit for bit, full-solve identity "
+                   "with the reference is unproven"}
     return rec, u, it
 
 
