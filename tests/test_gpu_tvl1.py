@@ -81,7 +81,10 @@ def test_zoom_out_bitexact(h, w, factor):
 
 
 @pytest.mark.parametrize("src,dst", [((120, 68), (240, 135)), ((17, 13), (33, 25)),
-                                     ((960, 540), (1920, 1080))])
+                                     ((960, 540), (1920, 1080)),
+                                     # factors 1.33 and 4, 1 (integer taps) and below 1
+                                     ((300, 200), (400, 267)), ((50, 30), (200, 120)),
+                                     ((70, 9), (70, 9)), ((400, 300), (130, 90))])
 def test_zoom_in_bitexact(src, dst):
     rng = np.random.default_rng(3)
     img = (rng.standard_normal((src[1], src[0])) * 4).astype(np.float32)
