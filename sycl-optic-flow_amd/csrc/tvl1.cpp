@@ -96,6 +96,7 @@ void release(of_tvl1* h)
     dev_free(h->d_iters);
     host_free(h->hflow);
     host_free(h->hstatus);
+    host_free(h->hreport);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : h->scale_ev) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -374,16 +375,18 @@ int solve_levels_once(of_tvl1* h, const float* init, int first, const int* warps
         }
         if (h->scale_timing) OF_HIP(hipEventRecord(h->scale_ev[2 * s + 1], st));
     }
-    OF_TRY(launch_copy_flow(h->ubase, h->ubase + 2 * N0, dst, 2 * N0, h->ctl, 0, st));
-    OF_HIP(hipMemcpyAsync(h->h_iters.data(), h->d_iters, sizeof(int) * h->h_iters.size(),
-                          hipMemcpyDeviceToHost, st));
+    // the final copy also writes the iteration counts and the error word to
+    // host-mapped memory: one synchronisation, no copies back
+    const int n_iters = (int)h->h_iters.size();
+    OF_TRY(launch_copy_flow(h->ubase, h->ubase + 2 * N0, dst, 2 * N0, h->ctl, 0, st, h->d_iters,
+                            n_iters, h->hreport));
     OF_HIP(hipStreamSynchronize(st));
+    std::copy(h->hreport, h->hreport + n_iters, h->h_iters.begin());
     {
         // a persistent warping whose grid barrier ran past its bound flags
         // it; so does one whose grid did not fully start (the final copy
         // then skipped `dst`)
-        int derr = 0;
-        OF_HIP(hipMemcpy(&derr, &h->ctl->error, sizeof(int), hipMemcpyDeviceToHost));
+        const int derr = h->hreport[n_iters];
         if (derr == kBarNotResident) return kRetryHostPaced;
         if (derr) {
             record_hip_error(hipErrorLaunchTimeOut, "k_tvl1_persist grid barrier", __FILE__, __LINE__);
@@ -633,8 +636,12 @@ int of_tvl1_create(const of_tvl1_params* params, int width, int height, of_tvl1*
         if (!st) st = dev_alloc(&h->d_iters, (size_t)ns * p.warps);
         if (!st) st = host_alloc(&h->hflow, 2 * N0, hipHostMallocDefault);
         if (!st) st = host_alloc(&h->hstatus, 1, hipHostMallocMapped | hipHostMallocCoherent);
+        if (!st)
+            st = host_alloc(&h->hreport, (size_t)ns * p.warps + 1,
+                            hipHostMallocMapped | hipHostMallocCoherent);
         if (st) { release(h); return st; }
         *h->hstatus = 0ull;
+        std::memset(h->hreport, 0, sizeof(int) * ((size_t)ns * p.warps + 1));
         std::memset(h->hflow, 0, sizeof(float) * 2 * N0);
         h->h_iters.assign((size_t)ns * p.warps, 0);
         // flags carry the warping's seq (from 1): zero means never set

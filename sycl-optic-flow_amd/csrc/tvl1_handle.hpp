@@ -67,6 +67,7 @@ struct of_tvl1 {
     int run_first = -1;               // coarsest scale solved in the last solve
     int* d_iters = nullptr;
     unsigned long long* hstatus = nullptr;   // host-mapped
+    int* hreport = nullptr;   // host-mapped: a solve's iteration counts, then its error word
     unsigned epoch = 0;
     std::vector<int> h_iters;
     amdflow::GaussTaps g_pre{}, g_zoom{};

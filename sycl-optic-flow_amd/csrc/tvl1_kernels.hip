@@ -438,11 +438,21 @@ __global__ __launch_bounds__(256) void k_warp_setup(const float* __restrict__ I0
 }
 
 // Copies the finest scale's current flow into the caller's planar buffer.
+// With `report`, one thread also writes the solve's per-warping iteration
+// counts (`iters`, n_iters of them) and then its error word to
+// report[0..n_iters], host-mapped memory the host reads after its stream
+// synchronisation (no copy back).
 __global__ __launch_bounds__(256) void k_copy_flow(const float* __restrict__ ub0,
                                                    const float* __restrict__ ub1,
                                                    float* __restrict__ out, unsigned n2,
-                                                   const TvCtl* ctl, int level)
+                                                   const TvCtl* ctl, int level,
+                                                   const int* __restrict__ iters, int n_iters,
+                                                   int* __restrict__ report)
 {
+    if (report && blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < n_iters; k++) report[k] = iters[k];
+        report[n_iters] = ctl->error;
+    }
     if (ctl->error) return;   // a failed solve leaves the destination as it was
     const float* ub = ctl->cur[level] ? ub1 : ub0;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x)
@@ -701,10 +711,10 @@ int launch_warp_setup(const float* I0, const float* src4, const float* ub0, cons
 }
 
 int launch_copy_flow(const float* ub0, const float* ub1, float* out, size_t n2, const TvCtl* ctl,
-                     int level, hipStream_t s)
+                     int level, hipStream_t s, const int* iters, int n_iters, int* report)
 {
     hipLaunchKernelGGL(k_copy_flow, dim3(grid1d(n2)), dim3(256), 0, s, ub0, ub1, out,
-                       (unsigned)n2, ctl, level);
+                       (unsigned)n2, ctl, level, iters, n_iters, report);
     OF_LAUNCH_CHECK("k_copy_flow");
     return OF_OK;
 }

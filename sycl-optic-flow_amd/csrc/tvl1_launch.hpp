@@ -48,8 +48,11 @@ int launch_pack3(const float* a, const float* b, const float* c, float* out4, si
 int launch_warp_setup(const float* I0, const float* src4, const float* ub0, const float* ub1,
                       float* wx, float* wy, float* rhoc, int nx, int ny, TvCtl* ctl, int level,
                       hipStream_t s);
+// With `report` (host-mapped, n_iters + 1 ints), also writes the iteration
+// counts `iters` and the control block's error word there.
 int launch_copy_flow(const float* ub0, const float* ub1, float* out, size_t n2, const TvCtl* ctl,
-                     int level, hipStream_t s);
+                     int level, hipStream_t s, const int* iters = nullptr, int n_iters = 0,
+                     int* report = nullptr);
 int launch_energy(const float* I0, const float* I1, const float* u1, const float* u2, int nx,
                   int ny, float lambda, double* partials, int max_partials, double* out,
                   hipStream_t s);
