@@ -63,11 +63,11 @@ def _sources_hash() -> str:
 # --copy-too); the runs are summarised under profiles/.  Counters cannot be
 # read inside a timed run, so the value is carried here and reported only
 # when the same kernel dominates and the sources are the measured ones.
-MEASURED_TRAFFIC = {"sources": "df66f68b76252b72",
-                    "kernels": {"k_tvl1_persist_nbr<2,8>": 1834343607}}
+MEASURED_TRAFFIC = {"sources": "910c8df0a6e81b5f",
+                    "kernels": {"k_tvl1_persist_nbr<2,8>": 1833529189}}
 # DRAM bytes of one run of the 64-pair batch, measured the same way.
-MEASURED_BATCH_TRAFFIC = {"sources": "df66f68b76252b72", "pairs": 64, "seed0": BATCH_SEED0,
-                          "bytes": 241529758956}
+MEASURED_BATCH_TRAFFIC = {"sources": "910c8df0a6e81b5f", "pairs": 64, "seed0": BATCH_SEED0,
+                          "bytes": 241520440900}
 
 
 def parse():
